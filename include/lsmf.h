@@ -1,0 +1,208 @@
+/*
+ * lsmf.h -- C ABI of the MI355X matrix-free Neo-Hookean tangent operator and
+ * the GMG-preconditioned CG / Newton solve it drives.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository davydden/large-strain-matrix-free).  The
+ * reference's operator is the C++ template
+ *   NeoHookOperator<dim, fe_degree, n_q_points_1d, number>
+ * (include/mf_nh_operator.h:147-296); this ABI exposes the same member set
+ * through an opaque handle, with plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Status codes: 0 = OK, negative = error; lsmf_last_error() returns a
+ *     thread-local message for the last failing call on this thread.
+ *   - Vectors passed to vmult / diagonal / residual are DEVICE pointers,
+ *     borrowed for the duration of the call.  Their element type is the
+ *     operator's precision (double for LSMF_FP64, float for LSMF_FP32).
+ *   - Global DoF numbering: dof = dim * node + component.  Local node order
+ *     inside a cell is lexicographic (x fastest), as in deal.II's
+ *     FEEvaluation (include/mf_nh_operator.h:827-832 uses
+ *     j + c * dofs_per_component for the cell-local layout).
+ *   - All device work is stream-ordered on the stream argument (a
+ *     hipStream_t; NULL = default stream); vmult never synchronises the host.
+ *     One handle must not be used from two streams at once (the reference's
+ *     vmult is not thread-safe either: include/mf_elasticity.h:1755).
+ */
+#ifndef LSMF_H
+#define LSMF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int lsmf_status;
+#define LSMF_OK 0
+#define LSMF_ERR_INVALID_ARGUMENT (-1)
+#define LSMF_ERR_UNKNOWN_CACHING (-2) /* ExcMessage("Unknown caching"), include/mf_nh_operator.h:438-442 */
+#define LSMF_ERR_OUT_OF_MEMORY (-3)
+#define LSMF_ERR_NONPOSITIVE_DET_F (-4) /* Assert(det_F > 0), include/mf_nh_operator.h:497-503 */
+#define LSMF_ERR_NOT_CACHED (-5)        /* vmult before cache() */
+#define LSMF_ERR_HIP (-6)
+#define LSMF_ERR_NOT_CONVERGED (-7) /* "No convergence in nonlinear solver!", include/mf_elasticity.h:2557 */
+#define LSMF_ERR_UNSUPPORTED (-8)   /* (dim, degree, n_q_1d) not instantiated, src/main.cc:102-110 */
+
+typedef void *lsmf_stream; /* hipStream_t */
+
+/* MF caching variants, include/mf_nh_operator.h:284-293 ("MF caching" prm key,
+ * include/mf_elasticity.h:445-450). */
+enum {
+  LSMF_CACHING_SCALAR = 0,
+  LSMF_CACHING_TENSOR2 = 1,
+  LSMF_CACHING_TENSOR4 = 2,
+  LSMF_CACHING_TENSOR4_NS = 3,
+  LSMF_CACHING_SCALAR_REFERENTIAL = 4
+};
+
+/* operator number type: the reference instantiates NeoHookOperator with
+ * number = double on the fine level and float on the MG levels
+ * (include/mf_elasticity.h:673-674, 900-902). */
+enum { LSMF_FP64 = 0, LSMF_FP32 = 1 };
+
+/* vmult phase mask, the reference's MFMask (include/mf_nh_operator.h:73-86).
+ * ZERO and MPI have no effect on one GPU (dst is overwritten unless add != 0;
+ * there are no ghosts).  PLAIN is new: apply the operator without
+ * constraints (constrained DoFs are read and written like free ones), which
+ * the residual needs for the Dirichlet lifting -K g
+ * (include/mf_elasticity.h:2894-2898). */
+enum {
+  LSMF_MASK_ZERO = 1u << 0,
+  LSMF_MASK_MPI = 1u << 1,
+  LSMF_MASK_RW = 1u << 2,
+  LSMF_MASK_SF = 1u << 3,
+  LSMF_MASK_QD = 1u << 4,
+  LSMF_MASK_DEFAULT = 0x1fu,
+  LSMF_MASK_CELLLOOP = (1u << 2) | (1u << 3) | (1u << 4),
+  LSMF_MASK_RWSF = (1u << 2) | (1u << 3),
+  LSMF_MASK_PLAIN = 1u << 5
+};
+
+/* Mesh of one level.  Replaces the DoFHandler + MatrixFree index data the
+ * reference builds in Solid::system_setup / setup_matrix_free
+ * (include/mf_elasticity.h:1610-1708, 1752-1831). */
+typedef struct {
+  int dim;                    /* 2 or 3 */
+  int degree;                 /* FE_Q degree p */
+  int n_cells;
+  int n_nodes;                /* (p+1)^dim support points per cell are shared */
+  const int32_t *cell_nodes;  /* host, [n_cells][(p+1)^dim], lexicographic */
+  const double *coords;       /* host, [n_nodes][dim] reference coordinates of the
+                                 support points; cell geometry is the Q1 map of the
+                                 2^dim corner nodes (StaticMappingQ1,
+                                 include/mf_elasticity.h:1828-1829) */
+  const uint8_t *constrained; /* host, [n_nodes*dim], 1 = homogeneous Dirichlet
+                                 DoF of this operator, or NULL */
+  const int32_t *cell_colour; /* host, [n_cells] colouring with no node shared
+                                 inside a colour, or NULL (greedy colouring) */
+} lsmf_mesh_desc;
+
+typedef struct {
+  int n_q_1d;    /* QGauss points per direction (n_q_points_1d) */
+  int precision; /* LSMF_FP64 or LSMF_FP32 */
+  int caching;   /* LSMF_CACHING_* */
+} lsmf_op_options;
+
+typedef struct lsmf_op lsmf_op;
+
+/* NeoHookOperator::initialize (include/mf_nh_operator.h:394-443): binds the
+ * mesh and the caching mode and allocates the per-qp caches.  Uploads index
+ * tables, colouring and geometry to the current HIP device. */
+lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *opt, lsmf_op **out);
+
+/* NeoHookOperator::clear (include/mf_nh_operator.h:379-390) + destructor. */
+lsmf_status lsmf_op_destroy(lsmf_op *op);
+
+/* NeoHookOperator::set_material (include/mf_nh_operator.h:596-608): the
+ * reference passes a matrix and an inclusion Material selected by
+ * material_id (:930-932); here mu and lambda are per-cell host arrays
+ * [n_cells], in the mesh's cell order, so the two-material case is a special
+ * case.  lambda is Lame's first parameter, lambda = 2 mu nu / (1 - 2 nu)
+ * (include/material.h:82-94). */
+lsmf_status lsmf_set_material_per_cell(lsmf_op *op, const double *mu, const double *lambda);
+
+/* The displacement argument of NeoHookOperator::initialize
+ * (include/mf_nh_operator.h:397, 404): the linearisation state.  u_dev is a
+ * device FP64 vector [n_nodes*dim]; it is copied (and converted for FP32
+ * operators), so it need not outlive the call. */
+lsmf_status lsmf_set_linearization(lsmf_op *op, const double *u_dev, lsmf_stream stream);
+
+/* NeoHookOperator::cache (include/mf_nh_operator.h:447-592).  Kinematics
+ * (F, det F, ln J) are evaluated in FP64 for both precisions.  Returns
+ * LSMF_ERR_NONPOSITIVE_DET_F if any quadrature point has det F <= 0 (this
+ * call synchronises the stream to read that count). */
+lsmf_status lsmf_cache(lsmf_op *op, lsmf_stream stream);
+
+/* NeoHookOperator::compute_diagonal (include/mf_nh_operator.h:1503-1557):
+ * exact diagonal, constrained entries 1, and its inverse. */
+lsmf_status lsmf_compute_diagonal(lsmf_op *op, lsmf_stream stream);
+
+/* NeoHookOperator::vmult / vmult_add / Tvmult / Tvmult_add
+ * (include/mf_nh_operator.h:612-706): dst = A src (add = 0) or dst += A src
+ * (add != 0).  Constrained rows act as identity (:703-705) unless
+ * LSMF_MASK_PLAIN.  The operator is symmetric, so Tvmult = vmult.  dst and
+ * src must not alias. */
+lsmf_status lsmf_vmult(lsmf_op *op, void *dst, const void *src, int add, unsigned mask, lsmf_stream stream);
+
+/* get_matrix_diagonal_inverse() / el(i,i) (include/mf_nh_operator.h:206-211,
+ * 1561-1571): device pointers to the diagonal and its inverse (operator
+ * precision), valid until the next compute_diagonal or destroy. */
+lsmf_status lsmf_get_diagonal(lsmf_op *op, const void **diag_dev, const void **inv_diag_dev);
+
+/* precondition_Jacobi (include/mf_nh_operator.h:348-358): dst = omega D^-1 src */
+lsmf_status lsmf_precondition_jacobi(lsmf_op *op, void *dst, const void *src, double omega, lsmf_stream stream);
+
+/* m() / n() (include/mf_nh_operator.h:362-375) */
+size_t lsmf_op_size(const lsmf_op *op);
+
+/* memory_consumption (include/mf_nh_operator.h:300-328): bytes of device
+ * memory vmult reads (caches, index tables, geometry), diagonals excluded. */
+size_t lsmf_memory_consumption(const lsmf_op *op);
+
+/* FP64 internal-force residual, the cell part of Solid::assemble_system
+ * (include/mf_elasticity.h:2758-2804): rhs = - sum_cells int P : Grad N dV,
+ * P = tau F^-T, at the linearisation state set by lsmf_set_linearization.
+ * No constraints applied; rhs_dev is FP64 [n_nodes*dim].  The operator must
+ * be FP64. */
+lsmf_status lsmf_residual(lsmf_op *op, double *rhs_dev, lsmf_stream stream);
+
+/* thread-local description of the last error on this thread */
+const char *lsmf_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Structured meshes of the unit square / cube (and the Cook membrane used by
+ * the reference's tests).  Replaces GridGenerator::subdivided_hyper_rectangle
+ * + GridTools::transform + refine_global (include/mf_elasticity.h:1339-1398)
+ * for the geometries this library runs.  Host-only; no GPU needed.
+ * ---------------------------------------------------------------------- */
+enum { LSMF_GEOMETRY_BOX = 0, LSMF_GEOMETRY_COOK = 1 };
+
+typedef struct {
+  int dim, degree;
+  int n_cells_dir[3];
+  int n_cells, n_nodes;
+  int32_t *cell_nodes; /* [n_cells][(p+1)^dim] */
+  double *coords;      /* [n_nodes][dim] */
+  int32_t *cell_colour; /* [n_cells] parity colouring, 2^dim colours */
+} lsmf_mesh;
+
+/* Box [lo, hi] with n_cells_dir cells per direction; geometry COOK applies the
+ * reference's grid_y_transform (include/mf_elasticity.h:1252-1270) to the box
+ * and then scales all coordinates by `scale`. */
+lsmf_status lsmf_mesh_create(int dim, int degree, const int *n_cells_dir, const double *lo, const double *hi,
+                             int geometry, double scale, lsmf_mesh **out);
+lsmf_status lsmf_mesh_destroy(lsmf_mesh *mesh);
+/* side s in [0, 2*dim): face 2d (+0) is the low-x_d side, 2d+1 the high side,
+ * as deal.II's face numbering.  Writes (cell, local face) pairs; returns the
+ * count (call with NULL outputs to query). */
+int lsmf_mesh_boundary_faces(const lsmf_mesh *mesh, int side, int32_t *cells, int32_t *faces);
+/* nodes on side s; returns the count (NULL output to query) */
+int lsmf_mesh_boundary_nodes(const lsmf_mesh *mesh, int side, int32_t *nodes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSMF_H */

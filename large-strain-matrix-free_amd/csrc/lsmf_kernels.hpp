@@ -1,0 +1,885 @@
+// lsmf_kernels.hpp -- HIP kernels of the matrix-free Neo-Hookean tangent for
+// gfx950 (MI355X), one fused kernel per cell colour:
+//   gather (index table with constraint bits) -> LDS
+//   -> sum-factorized gradient (1D matrices, one thread per quadrature point)
+//   -> pointwise tangent (scalar / tensor2 / tensor4 caches, FP64 or FP32)
+//   -> sum-factorized integration -> colour-ordered scatter (no atomics).
+//
+// Reference behaviour being reproduced: NeoHookOperator::local_apply_cell and
+// do_operation_on_cell (include/mf_nh_operator.h:710-785, 922-1499), cache()
+// (:447-592) and compute_diagonal() (:1503-1557), formulation 1.
+//
+// Thread layout: N = p + 1 = n_q_1d, NQP = N^dim.  Lane r of a cell is both
+// the lexicographic support point r (gather / scatter) and the quadrature
+// point r (pointwise work), so no lane idles between phases.  A 256-lane
+// workgroup holds floor(256 / NQP) cells (Q2-3D: 9 cells, 243 lanes).
+//
+// Cache layout (structure of arrays, colour-sorted cells):
+//   field f of quadrature point r of sorted cell k at  f * fs + k * NQP + r,
+// so the NQP * cells_per_workgroup lanes of a workgroup read one contiguous
+// run per field.  Jacobian-weighted forms are stored (JxW folded in):
+//   tensor4 : K = J_cur^-1 (dim^2), T = tau det(J_ref) w (Voigt, 6),
+//             M = (2c S + 2 lambda IxI) det(J_ref) w (Voigt 6x6 upper, 21)
+//   tensor2 : K, T, c1 = 2c det(J_ref) w, c2 = 2 lambda det(J_ref) w
+//   scalar  : c = mu - 2 lambda ln J   (F, J_ref recomputed per vmult)
+// With these, flux = Q K^T where Q = g T + M : sym(g) (or the tensor2 form)
+// and g = Grad_xi v K, i.e. exactly the reference's
+// Q = [g tau + Jc : sym g] / J integrated with JxW_cur = J det(J_ref) w.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lsmf {
+
+constexpr int kWG = 256;
+constexpr uint32_t kNodeMask = (1u << 28) - 1u;
+constexpr int kConsShift = 28;  // bits 28..30: component c constrained
+constexpr uint32_t kFirstBit = 1u << 31;  // this cell writes (not adds) first
+
+enum Variant { V_SCALAR = 0, V_TENSOR2 = 1, V_TENSOR4 = 2, V_TENSOR4_NS = 3, V_SCALAR_REF = 4, V_RESIDUAL = 100 };
+
+// phase bits (reference MFMask, include/mf_nh_operator.h:73-86)
+constexpr unsigned PH_SF = 1u << 3;
+constexpr unsigned PH_QD = 1u << 4;
+
+__host__ __device__ constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+
+template <int DIM> struct Voigt;
+template <> struct Voigt<2> {
+  static constexpr int NV = 3;
+};
+template <> struct Voigt<3> {
+  static constexpr int NV = 6;
+};
+template <int DIM> __host__ __device__ constexpr int voigt_first(int v) {
+  return DIM == 3 ? (v < 3 ? v : (v == 5 ? 1 : 0)) : (v < 2 ? v : 0);
+}
+template <int DIM> __host__ __device__ constexpr int voigt_second(int v) {
+  return DIM == 3 ? (v < 3 ? v : (v == 3 ? 1 : 2)) : (v < 2 ? v : 1);
+}
+template <int DIM> __host__ __device__ constexpr int voigt_index(int i, int j) {
+  return i == j ? i : (DIM == 2 ? 2 : (i + j == 1 ? 3 : (i + j == 2 ? 4 : 5)));
+}
+// packed upper triangle of an NV x NV symmetric matrix
+template <int NV> __host__ __device__ constexpr int sym_index(int v, int w) {
+  return v <= w ? v * NV - v * (v - 1) / 2 + (w - v) : w * NV - w * (w - 1) / 2 + (v - w);
+}
+
+template <int DIM, int VAR> struct CacheFields {
+  static constexpr int NV = Voigt<DIM>::NV;
+  static constexpr int value = VAR == V_TENSOR4   ? DIM * DIM + NV + NV * (NV + 1) / 2
+                               : VAR == V_TENSOR2 ? DIM * DIM + NV + 2
+                               : VAR == V_SCALAR  ? 1
+                                                  : 0;
+};
+
+template <int DIM, int N> struct CellShape {
+  static constexpr int NQP = ipow(N, DIM);
+  static constexpr int CPW = kWG / NQP;  // cells per workgroup
+  static constexpr int NB = DIM * DIM;   // arrays per LDS ping-pong buffer
+  static constexpr int NC = 1 << DIM;    // Q1 corners
+};
+
+// LDS-only workgroup barrier: waits for this wave's LDS traffic, not for its
+// outstanding global loads, so the per-qp cache loads issued at kernel entry
+// stay in flight across the sum-factorization stages.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
+// XCD processes a contiguous range of cells (consecutive workgroups share
+// cache lines of the SoA caches and of the vectors at their boundaries).
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  const int per = nblocks / 8;
+  if (bid >= per * 8) return bid;
+  return (bid % 8) * per + bid / 8;
+}
+
+template <typename T> struct Tables {
+  // S[q][a], D[q][a] (forward), St[a][q] = S[q][a] stored row-major by a, Dt
+  // plus Gauss points and weights on [0,1]
+  const T *S;  // layout in device memory: S, D, St, Dt each N*N, then qp[N], qw[N]
+};
+
+template <int N, int STRIDE, typename T>
+__device__ __forceinline__ void load_line(const T *__restrict__ in, int base, T (&x)[N]) {
+#pragma unroll
+  for (int a = 0; a < N; ++a) x[a] = in[base + a * STRIDE];
+}
+template <int N, typename T> __device__ __forceinline__ T dotN(const T *__restrict__ row, const T (&x)[N]) {
+  T s = row[0] * x[0];
+#pragma unroll
+  for (int a = 1; a < N; ++a) s = fma(row[a], x[a], s);
+  return s;
+}
+
+// ------------------------------------------------------------------------
+// small tensor helpers (row-major DIM x DIM)
+// ------------------------------------------------------------------------
+template <int DIM, typename T> __device__ __forceinline__ T det(const T (&A)[DIM][DIM]) {
+  if constexpr (DIM == 2) return A[0][0] * A[1][1] - A[0][1] * A[1][0];
+  else
+    return A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+           A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+}
+// B = A^-1, returns det(A)
+template <int DIM, typename T> __device__ __forceinline__ T inverse(const T (&A)[DIM][DIM], T (&B)[DIM][DIM]) {
+  if constexpr (DIM == 2) {
+    const T d = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    const T id = T(1) / d;
+    B[0][0] = A[1][1] * id;
+    B[0][1] = -A[0][1] * id;
+    B[1][0] = -A[1][0] * id;
+    B[1][1] = A[0][0] * id;
+    return d;
+  } else {
+    const T c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    const T c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    const T c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    const T d = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+    const T id = T(1) / d;
+    B[0][0] = c00 * id;
+    B[1][0] = c01 * id;
+    B[2][0] = c02 * id;
+    B[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * id;
+    B[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * id;
+    B[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * id;
+    B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
+    B[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
+    B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
+    return d;
+  }
+}
+
+// ------------------------------------------------------------------------
+// sum factorization on one cell's LDS slot (lane r, multi-index i0,i1,i2)
+// ------------------------------------------------------------------------
+template <int DIM, int N, typename T> struct SF {
+  static constexpr int NQP = ipow(N, DIM);
+  // tab: S, D, St, Dt (N*N each) in LDS
+
+  // nodal values of NCOMP components in A[c*NQP + .] -> gradients G[c][d] in
+  // registers; B is scratch.  Ends with data in registers; caller must
+  // lds_barrier() before the next writes to A.
+  template <int NCOMP>
+  static __device__ __forceinline__ void evaluate(const T *tab, T *A, T *B, bool on, int r, int i0, int i1, int i2,
+                                                  T (&G)[NCOMP][DIM]) {
+    const T *S = tab, *D = tab + N * N;
+    // x direction: A -> B (value, derivative)
+    if (on) {
+      const int base = r - i0;
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        T x[N];
+        load_line<N, 1>(A + c * NQP, base, x);
+        B[(2 * c) * NQP + r] = dotN<N>(S + i0 * N, x);
+        B[(2 * c + 1) * NQP + r] = dotN<N>(D + i0 * N, x);
+      }
+    }
+    lds_barrier();
+    if constexpr (DIM == 2) {
+      const int base = on ? r - i1 * N : 0;
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        T x0[N], x1[N];
+        load_line<N, N>(B + (2 * c) * NQP, base, x0);
+        load_line<N, N>(B + (2 * c + 1) * NQP, base, x1);
+        G[c][0] = dotN<N>(S + i1 * N, x1);
+        G[c][1] = dotN<N>(D + i1 * N, x0);
+      }
+    } else {
+      if (on) {
+        const int base = r - i1 * N;
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) {
+          T x0[N], x1[N];
+          load_line<N, N>(B + (2 * c) * NQP, base, x0);
+          load_line<N, N>(B + (2 * c + 1) * NQP, base, x1);
+          A[(3 * c + 0) * NQP + r] = dotN<N>(S + i1 * N, x0);  // value in x,y
+          A[(3 * c + 1) * NQP + r] = dotN<N>(S + i1 * N, x1);  // d/dx
+          A[(3 * c + 2) * NQP + r] = dotN<N>(D + i1 * N, x0);  // d/dy
+        }
+      }
+      lds_barrier();
+      const int base = on ? r - i2 * N * N : 0;
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        T y0[N], y1[N], y2[N];
+        load_line<N, N * N>(A + (3 * c + 0) * NQP, base, y0);
+        load_line<N, N * N>(A + (3 * c + 1) * NQP, base, y1);
+        load_line<N, N * N>(A + (3 * c + 2) * NQP, base, y2);
+        G[c][0] = dotN<N>(S + i2 * N, y1);
+        G[c][1] = dotN<N>(S + i2 * N, y2);
+        G[c][2] = dotN<N>(D + i2 * N, y0);
+      }
+    }
+  }
+
+  // fluxes Phi[c][d] (registers) -> out[c] = sum_q sum_d dphi_r/dxi_d Phi_cd.
+  // The buffer written first is W (must not be read by anyone after the last
+  // barrier), the other is scratch.  3D: W=B, then A, B.  2D: W=A, then B.
+  static __device__ __forceinline__ void integrate(const T *tab, T *W, T *O, bool on, int r, int i0, int i1, int i2,
+                                                   const T (&Phi)[DIM][DIM], T (&out)[DIM]) {
+    const T *St = tab + 2 * N * N, *Dt = tab + 3 * N * N;
+    if (on)
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) W[(DIM * c + d) * NQP + r] = Phi[c][d];
+    lds_barrier();
+    if constexpr (DIM == 3) {
+      if (on) {
+        const int base = r - i2 * N * N;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          T f0[N], f1[N], f2[N];
+          load_line<N, N * N>(W + (3 * c + 0) * NQP, base, f0);
+          load_line<N, N * N>(W + (3 * c + 1) * NQP, base, f1);
+          load_line<N, N * N>(W + (3 * c + 2) * NQP, base, f2);
+          O[(3 * c + 0) * NQP + r] = dotN<N>(St + i2 * N, f0);
+          O[(3 * c + 1) * NQP + r] = dotN<N>(St + i2 * N, f1);
+          O[(3 * c + 2) * NQP + r] = dotN<N>(Dt + i2 * N, f2);
+        }
+      }
+      lds_barrier();
+      if (on) {
+        const int base = r - i1 * N;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          T r10[N], r01[N], r00[N];
+          load_line<N, N>(O + (3 * c + 0) * NQP, base, r10);
+          load_line<N, N>(O + (3 * c + 1) * NQP, base, r01);
+          load_line<N, N>(O + (3 * c + 2) * NQP, base, r00);
+          W[(2 * c + 0) * NQP + r] = dotN<N>(St + i1 * N, r00) + dotN<N>(Dt + i1 * N, r01);
+          W[(2 * c + 1) * NQP + r] = dotN<N>(St + i1 * N, r10);
+        }
+      }
+      lds_barrier();
+      {
+        const int base = on ? r - i0 : 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          T p0[N], p1[N];
+          load_line<N, 1>(W + (2 * c + 0) * NQP, base, p0);
+          load_line<N, 1>(W + (2 * c + 1) * NQP, base, p1);
+          out[c] = dotN<N>(St + i0 * N, p0) + dotN<N>(Dt + i0 * N, p1);
+        }
+      }
+    } else {
+      if (on) {
+        const int base = r - i1 * N;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          T fx[N], fy[N];
+          load_line<N, N>(W + (2 * c + 0) * NQP, base, fx);
+          load_line<N, N>(W + (2 * c + 1) * NQP, base, fy);
+          O[(2 * c + 0) * NQP + r] = dotN<N>(Dt + i1 * N, fy);
+          O[(2 * c + 1) * NQP + r] = dotN<N>(St + i1 * N, fx);
+        }
+      }
+      lds_barrier();
+      {
+        const int base = on ? r - i0 : 0;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          T p0[N], p1[N];
+          load_line<N, 1>(O + (2 * c + 0) * NQP, base, p0);
+          load_line<N, 1>(O + (2 * c + 1) * NQP, base, p1);
+          out[c] = dotN<N>(St + i0 * N, p0) + dotN<N>(Dt + i0 * N, p1);
+        }
+      }
+    }
+  }
+};
+
+// ------------------------------------------------------------------------
+// pointwise tangent actions (reference: include/mf_nh_operator.h:1182-1440)
+// ------------------------------------------------------------------------
+// g = G K
+template <int DIM, typename T>
+__device__ __forceinline__ void grad_times(const T (&G)[DIM][DIM], const T (&K)[DIM][DIM], T (&g)[DIM][DIM]) {
+#pragma unroll
+  for (int i = 0; i < DIM; ++i)
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+      T s = G[i][0] * K[0][j];
+#pragma unroll
+      for (int d = 1; d < DIM; ++d) s = fma(G[i][d], K[d][j], s);
+      g[i][j] = s;
+    }
+}
+// Phi = Q K^T
+template <int DIM, typename T>
+__device__ __forceinline__ void flux_times_kt(const T (&Q)[DIM][DIM], const T (&K)[DIM][DIM], T (&Phi)[DIM][DIM]) {
+#pragma unroll
+  for (int c = 0; c < DIM; ++c)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      T s = Q[c][0] * K[d][0];
+#pragma unroll
+      for (int e = 1; e < DIM; ++e) s = fma(Q[c][e], K[d][e], s);
+      Phi[c][d] = s;
+    }
+}
+
+// tensor4: Q = g T + M : sym(g)   (cached T, M already carry JxW)
+template <int DIM, typename T>
+__device__ __forceinline__ void qp_tensor4(const T (&G)[DIM][DIM], const T *cch, T (&Phi)[DIM][DIM]) {
+  constexpr int NV = Voigt<DIM>::NV;
+  T K[DIM][DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) K[d][j] = cch[d * DIM + j];
+  const T *Tv = cch + DIM * DIM;
+  const T *M = Tv + NV;
+  T g[DIM][DIM];
+  grad_times<DIM>(G, K, g);
+  T sv[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const int i = voigt_first<DIM>(v), j = voigt_second<DIM>(v);
+    sv[v] = (i == j) ? g[i][i] : g[i][j] + g[j][i];
+  }
+  T rv[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    T s = M[sym_index<NV>(v, 0)] * sv[0];
+#pragma unroll
+    for (int w = 1; w < NV; ++w) s = fma(M[sym_index<NV>(v, w)], sv[w], s);
+    rv[v] = s;
+  }
+  T Q[DIM][DIM];
+#pragma unroll
+  for (int i = 0; i < DIM; ++i)
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+      T s = rv[voigt_index<DIM>(i, j)];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) s = fma(g[i][k], Tv[voigt_index<DIM>(k, j)], s);
+      Q[i][j] = s;
+    }
+  flux_times_kt<DIM>(Q, K, Phi);
+}
+
+// tensor2: Q = c1 sym(g) + c2 tr(g) I + g T
+template <int DIM, typename T>
+__device__ __forceinline__ void qp_tensor2(const T (&G)[DIM][DIM], const T *cch, T (&Phi)[DIM][DIM]) {
+  constexpr int NV = Voigt<DIM>::NV;
+  T K[DIM][DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) K[d][j] = cch[d * DIM + j];
+  const T *Tv = cch + DIM * DIM;
+  const T c1 = cch[DIM * DIM + NV], c2 = cch[DIM * DIM + NV + 1];
+  T g[DIM][DIM];
+  grad_times<DIM>(G, K, g);
+  T tr = g[0][0];
+#pragma unroll
+  for (int d = 1; d < DIM; ++d) tr += g[d][d];
+  const T hc1 = T(0.5) * c1;
+  T Q[DIM][DIM];
+#pragma unroll
+  for (int i = 0; i < DIM; ++i)
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+      T s = hc1 * (g[i][j] + g[j][i]);
+      if (i == j) s = fma(c2, tr, s);
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) s = fma(g[i][k], Tv[voigt_index<DIM>(k, j)], s);
+      Q[i][j] = s;
+    }
+  flux_times_kt<DIM>(Q, K, Phi);
+}
+
+// Q1 reference Jacobian at Gauss point (i0,i1,i2) from the cell's corner
+// coordinates X[v][i] (LDS); returns J_ref[i][d] = dX_i / dxi_d.
+template <int DIM, typename T, typename TC>
+__device__ __forceinline__ void q1_jacobian(const TC *X, const T *qp, int i0, int i1, int i2, T (&J)[DIM][DIM]) {
+  const T xi[3] = {qp[i0], qp[i1], DIM == 3 ? qp[i2] : T(0)};
+#pragma unroll
+  for (int i = 0; i < DIM; ++i)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) J[i][d] = T(0);
+#pragma unroll
+  for (int v = 0; v < (1 << DIM); ++v) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      T g = T(1);
+#pragma unroll
+      for (int e = 0; e < DIM; ++e) {
+        const int ve = (v >> e) & 1;
+        if (e == d) g *= ve ? T(1) : T(-1);
+        else g *= ve ? xi[e] : T(1) - xi[e];
+      }
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) J[i][d] = fma(T(X[v * DIM + i]), g, J[i][d]);
+    }
+  }
+}
+
+// local node index of Q1 corner v
+template <int DIM, int N> __host__ __device__ constexpr int corner_node(int v) {
+  return ((v & 1) ? N - 1 : 0) + ((v >> 1 & 1) ? (N - 1) * N : 0) + (DIM == 3 && (v >> 2 & 1) ? (N - 1) * N * N : 0);
+}
+// corner index of local node r, or -1
+template <int DIM, int N> __device__ __forceinline__ int node_corner(int i0, int i1, int i2) {
+  const bool e0 = (i0 == 0 || i0 == N - 1), e1 = (i1 == 0 || i1 == N - 1), e2 = (DIM == 2) || (i2 == 0 || i2 == N - 1);
+  if (!(e0 && e1 && e2)) return -1;
+  return (i0 ? 1 : 0) + (i1 ? 2 : 0) + ((DIM == 3 && i2) ? 4 : 0);
+}
+
+template <typename Number> struct VmultParams {
+  const uint32_t *__restrict__ idx;   // [cells][NQP] sorted cells
+  const Number *__restrict__ cache;   // SoA caches
+  size_t fs;                          // field stride = n_cells * NQP
+  const Number *__restrict__ tab;     // S, D, St, Dt, qp, qw (Number)
+  const Number *__restrict__ src;
+  Number *__restrict__ dst;
+  const Number *__restrict__ u;       // scalar / residual: displacement
+  const double *__restrict__ coords;  // scalar / residual: node coordinates
+  const Number *__restrict__ mu;      // per sorted cell
+  const Number *__restrict__ lambda;
+  int cell_begin, cell_end;
+};
+
+// Kinematic prologue shared by the scalar vmult and the residual: gathers u
+// and the corner coordinates, returns F = I + Grad_xi u J_ref^-1 and
+// J_ref^-1, det(J_ref) at this lane's quadrature point.  Leaves A,B free.
+template <int DIM, int N, typename T>
+__device__ __forceinline__ void kinematics_prologue(const VmultParams<T> &P, const T *tab, T *A, T *B, T *Xc,
+                                                    bool on, bool active, uint32_t code, int r, int i0, int i1, int i2,
+                                                    T (&F)[DIM][DIM], T (&Jri)[DIM][DIM], T &detJr) {
+  constexpr int NQP = ipow(N, DIM);
+  const int node = code & kNodeMask;
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) A[c * NQP + r] = P.u[(size_t)node * DIM + c];
+    const int v = node_corner<DIM, N>(i0, i1, i2);
+    if (v >= 0)
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) Xc[v * DIM + c] = T(P.coords[(size_t)node * DIM + c]);
+  }
+  lds_barrier();
+  T Gu[DIM][DIM];
+  SF<DIM, N, T>::template evaluate<DIM>(tab, A, B, on, r, i0, i1, i2, Gu);
+  T Jr[DIM][DIM];
+  q1_jacobian<DIM, T, T>(Xc, tab + 4 * N * N, i0, i1, i2, Jr);
+  detJr = inverse<DIM>(Jr, Jri);
+  grad_times<DIM>(Gu, Jri, F);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) F[d][d] += T(1);
+  lds_barrier();  // all lanes done reading A/B/Xc of this prologue
+}
+
+// ------------------------------------------------------------------------
+// fused vmult kernel, one launch per colour
+//   VAR: V_SCALAR / V_TENSOR2 / V_TENSOR4 / V_RESIDUAL
+//   ADD: dst += A src for every touch; otherwise the first-touch cell of each
+//        DoF stores (no separate zeroing pass)
+//   CONSTR: constrained DoFs read as zero and are skipped on write
+//   PH: phase bits (SF, QD) for breakdown timings
+// ------------------------------------------------------------------------
+template <int DIM, int N, typename T, int VAR, bool ADD, bool CONSTR, unsigned PH>
+__global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
+  using CS = CellShape<DIM, N>;
+  constexpr int NQP = CS::NQP, CPW = CS::CPW, NB = CS::NB, NC = CS::NC;
+  constexpr int NF = CacheFields<DIM, (VAR == V_RESIDUAL ? V_SCALAR : VAR)>::value;
+  constexpr bool KIN = (VAR == V_SCALAR || VAR == V_RESIDUAL);
+  constexpr int TABN = 4 * N * N + 2 * N;
+  __shared__ T lds[CPW * 2 * NB * NQP + TABN + (KIN ? CPW * NC * DIM : 0)];
+  T *tab = lds + CPW * 2 * NB * NQP;
+  T *Xall = tab + TABN;
+
+  const int t = threadIdx.x;
+  if (t < TABN) tab[t] = P.tab[t];
+  const int slot = t / NQP;
+  const int r = t - slot * NQP;
+  const int i0 = r % N, i1 = (r / N) % N, i2 = (DIM == 3) ? r / (N * N) : 0;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int cell = P.cell_begin + L * CPW + slot;
+  const bool on = slot < CPW;
+  const bool active = on && (cell < P.cell_end);
+  T *A = lds + (slot < CPW ? slot : 0) * 2 * NB * NQP;
+  T *B = A + NB * NQP;
+  T *Xc = Xall + (slot < CPW ? slot : 0) * NC * DIM;
+
+  // per-qp cache loads first: they stay in flight through the SF stages
+  T cch[NF > 0 ? NF : 1];
+  if constexpr ((PH & PH_QD) && NF > 0 && VAR != V_RESIDUAL) {
+    if (active) {
+      const T *cp = P.cache + (size_t)cell * NQP + r;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) cch[f] = cp[(size_t)f * P.fs];
+    }
+  }
+  uint32_t code = 0;
+  if (active) code = P.idx[(size_t)cell * NQP + r];
+  const size_t node = code & kNodeMask;
+
+  T F[DIM][DIM], Jri[DIM][DIM], detJr = T(1);
+  if constexpr (KIN && (PH & PH_QD)) {
+    kinematics_prologue<DIM, N, T>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr);
+  }
+
+  T G[DIM][DIM];
+  T out[DIM];
+  if constexpr (VAR != V_RESIDUAL) {
+    // gather src with constraints (read_dof_values, include/mf_nh_operator.h:745-749)
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        T v = P.src[node * DIM + c];
+        if (CONSTR && ((code >> (kConsShift + c)) & 1u)) v = T(0);
+        A[c * NQP + r] = v;
+      }
+    }
+    lds_barrier();
+    if constexpr (!(PH & PH_SF)) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) out[c] = A[c * NQP + r];
+    } else {
+      SF<DIM, N, T>::template evaluate<DIM>(tab, A, B, on, r, i0, i1, i2, G);
+    }
+  }
+
+  if constexpr ((PH & PH_SF) || VAR == V_RESIDUAL) {
+    T Phi[DIM][DIM];
+    if constexpr (!(PH & PH_QD)) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) Phi[c][d] = G[c][d];
+    } else if constexpr (VAR == V_TENSOR4) {
+      qp_tensor4<DIM, T>(G, cch, Phi);
+    } else if constexpr (VAR == V_TENSOR2) {
+      qp_tensor2<DIM, T>(G, cch, Phi);
+    } else {
+      // scalar (include/mf_nh_operator.h:1182-1254) or residual
+      // (include/mf_elasticity.h:2769-2804); kinematics recomputed here
+      const T m = active ? P.mu[cell] : T(0), l = active ? P.lambda[cell] : T(0);
+      T Fi[DIM][DIM];
+      const T J = inverse<DIM>(F, Fi);
+      // quadrature weight from the table (qw after qp)
+      const T *qw = tab + 4 * N * N + N;
+      const T w = (DIM == 3) ? qw[i0] * qw[i1] * qw[i2] : qw[i0] * qw[i1];
+      const T dw = detJr * w;
+      T K[DIM][DIM];  // J_cur^-1 = J_ref^-1 F^-1
+      grad_times<DIM>(Jri, Fi, K);
+      T tau[DIM][DIM];
+      const T cs = (VAR == V_RESIDUAL) ? (m - T(2) * l * T(log(double(J)))) : (active ? cch[0] : T(0));
+#pragma unroll
+      for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) {
+          T s = F[i][0] * F[j][0];
+#pragma unroll
+          for (int k = 1; k < DIM; ++k) s = fma(F[i][k], F[j][k], s);
+          tau[i][j] = m * s - (i == j ? cs : T(0));
+        }
+      if constexpr (VAR == V_RESIDUAL) {
+        // flux = -(tau F^-T) J_ref^-T det(J_ref) w = -tau K^T dw
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            T s = tau[c][0] * K[d][0];
+#pragma unroll
+            for (int e = 1; e < DIM; ++e) s = fma(tau[c][e], K[d][e], s);
+            Phi[c][d] = -s * dw;
+          }
+      } else {
+        T g[DIM][DIM];
+        grad_times<DIM>(G, K, g);
+        T tr = g[0][0];
+#pragma unroll
+        for (int d = 1; d < DIM; ++d) tr += g[d][d];
+        T Q[DIM][DIM];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i)
+#pragma unroll
+          for (int j = 0; j < DIM; ++j) {
+            T s = cs * (g[i][j] + g[j][i]);
+            if (i == j) s = fma(T(2) * l, tr, s);
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) s = fma(g[i][k], tau[k][j], s);
+            Q[i][j] = s * dw;
+          }
+        flux_times_kt<DIM>(Q, K, Phi);
+      }
+    }
+    if constexpr (DIM == 3)
+      SF<DIM, N, T>::integrate(tab, B, A, on, r, i0, i1, i2, Phi, out);
+    else
+      SF<DIM, N, T>::integrate(tab, A, B, on, r, i0, i1, i2, Phi, out);
+  }
+
+  // colour-ordered scatter (distribute_local_to_global, :769-780)
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      if (CONSTR && ((code >> (kConsShift + c)) & 1u)) continue;
+      T *d = P.dst + node * DIM + c;
+      if (!ADD && (code & kFirstBit)) *d = out[c];
+      else *d += out[c];
+    }
+  }
+}
+
+// constrained rows act as identity (include/mf_nh_operator.h:703-705)
+template <typename T>
+__global__ void k_constrained_identity(const int32_t *__restrict__ list, int n, const T *__restrict__ src,
+                                       T *__restrict__ dst, int add) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int k = list[i];
+    dst[k] = add ? dst[k] + src[k] : src[k];
+  }
+}
+
+// ------------------------------------------------------------------------
+// cache kernel (include/mf_nh_operator.h:447-592, formulation 1): FP64
+// kinematics for both storage precisions; counts det F <= 0.
+// ------------------------------------------------------------------------
+struct CacheParams {
+  const uint32_t *__restrict__ idx;
+  const double *__restrict__ tab;  // S, D, St, Dt, qp, qw (double)
+  const double *__restrict__ u;
+  const double *__restrict__ coords;
+  const double *__restrict__ mu;
+  const double *__restrict__ lambda;
+  void *cache;
+  size_t fs;
+  int n_cells;
+  int *bad;
+};
+
+template <int DIM, int N, typename T, int VAR> __global__ void __launch_bounds__(kWG) k_cache(const CacheParams P) {
+  using CS = CellShape<DIM, N>;
+  constexpr int NQP = CS::NQP, CPW = CS::CPW, NB = CS::NB, NC = CS::NC;
+  constexpr int NV = Voigt<DIM>::NV;
+  constexpr int TABN = 4 * N * N + 2 * N;
+  __shared__ double lds[CPW * 2 * NB * NQP + TABN + CPW * NC * DIM];
+  double *tab = lds + CPW * 2 * NB * NQP;
+  const int t = threadIdx.x;
+  if (t < TABN) tab[t] = P.tab[t];
+  const int slot = t / NQP, r = t - slot * NQP;
+  const int i0 = r % N, i1 = (r / N) % N, i2 = (DIM == 3) ? r / (N * N) : 0;
+  const int cell = blockIdx.x * CPW + slot;
+  const bool on = slot < CPW;
+  const bool active = on && (cell < P.n_cells);
+  double *A = lds + (slot < CPW ? slot : 0) * 2 * NB * NQP;
+  double *B = A + NB * NQP;
+  double *Xc = tab + TABN + (slot < CPW ? slot : 0) * NC * DIM;
+  uint32_t code = 0;
+  if (active) code = P.idx[(size_t)cell * NQP + r];
+  VmultParams<double> KP{};
+  KP.u = P.u;
+  KP.coords = P.coords;
+  double F[DIM][DIM], Jri[DIM][DIM], detJr;
+  kinematics_prologue<DIM, N, double>(KP, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr);
+  if (!active) return;
+  const double m = P.mu[cell], l = P.lambda[cell];
+  double Fi[DIM][DIM];
+  const double J = inverse<DIM>(F, Fi);
+  if (!(J > 0.0)) {
+    atomicAdd(P.bad, 1);
+  }
+  const double cs = m - 2.0 * l * log(J);
+  const double *qw = tab + 4 * N * N + N;
+  const double w = (DIM == 3) ? qw[i0] * qw[i1] * qw[i2] : qw[i0] * qw[i1];
+  const double dw = detJr * w;
+  double K[DIM][DIM];
+  grad_times<DIM>(Jri, Fi, K);
+  T *out = static_cast<T *>(P.cache) + (size_t)cell * NQP + r;
+  auto put = [&](int f, double v) { out[(size_t)f * P.fs] = T(v); };
+  if constexpr (VAR == V_SCALAR) {
+    put(0, cs);
+  } else {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) put(d * DIM + j, K[d][j]);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int i = voigt_first<DIM>(v), j = voigt_second<DIM>(v);
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) s += F[i][k] * F[j][k];
+      const double tau = m * s - (i == j ? cs : 0.0);
+      put(DIM * DIM + v, tau * dw);
+    }
+    if constexpr (VAR == V_TENSOR2) {
+      put(DIM * DIM + NV, 2.0 * cs * dw);
+      put(DIM * DIM + NV + 1, 2.0 * l * dw);
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int w2 = v; w2 < NV; ++w2) {
+          const int i = voigt_first<DIM>(v), j = voigt_second<DIM>(v);
+          const int k = voigt_first<DIM>(w2), ll = voigt_second<DIM>(w2);
+          const double Sijkl = 0.5 * (double(i == k && j == ll) + double(i == ll && j == k));
+          const double IxI = (i == j && k == ll) ? 1.0 : 0.0;
+          put(DIM * DIM + NV + sym_index<NV>(v, w2), (2.0 * cs * Sijkl + 2.0 * l * IxI) * dw);
+        }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// exact diagonal (include/mf_nh_operator.h:862-918 probes unit vectors; the
+// same entries in closed form): for local DoF (a, c) with
+// gamma = K^T grad_xi phi_a(q):
+//   A_(a,c),(a,c) = sum_q gamma.T.gamma + s~^T M s~   (tensor4)
+//                 = sum_q gamma.T.gamma + c1/2 (|gamma|^2 + gamma_c^2)
+//                          + c2 gamma_c^2               (tensor2 / scalar)
+// where s~ is the Voigt vector of sym(e_c (x) gamma) with doubled shears.
+// ------------------------------------------------------------------------
+template <int DIM, int N, typename T, int VAR>
+__global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
+  using CS = CellShape<DIM, N>;
+  constexpr int NQP = CS::NQP, CPW = CS::CPW, NB = CS::NB, NC = CS::NC;
+  constexpr int NV = Voigt<DIM>::NV;
+  constexpr int NF = (VAR == V_TENSOR4) ? CacheFields<DIM, V_TENSOR4>::value : CacheFields<DIM, V_TENSOR2>::value;
+  constexpr int TABN = 4 * N * N + 2 * N;
+  constexpr int SFN = (VAR == V_SCALAR) ? CPW * 2 * NB * NQP : 0;
+  __shared__ T lds[CPW * NF * NQP + TABN + SFN + (VAR == V_SCALAR ? CPW * NC * DIM : 0)];
+  T *qd_all = lds;
+  T *tab = lds + CPW * NF * NQP;
+  T *sf_all = tab + TABN;
+  T *X_all = sf_all + SFN;
+  const int t = threadIdx.x;
+  if (t < TABN) tab[t] = P.tab[t];
+  const int slot = t / NQP, r = t - slot * NQP;
+  const int i0 = r % N, i1 = (r / N) % N, i2 = (DIM == 3) ? r / (N * N) : 0;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int cell = P.cell_begin + L * CPW + slot;
+  const bool on = slot < CPW;
+  const bool active = on && (cell < P.cell_end);
+  const int sl = slot < CPW ? slot : 0;
+  T *qd = qd_all + sl * NF * NQP;
+  uint32_t code = 0;
+  if (active) code = P.idx[(size_t)cell * NQP + r];
+  if constexpr (VAR == V_SCALAR) {
+    T *A = sf_all + sl * 2 * NB * NQP;
+    T *B = A + NB * NQP;
+    T *Xc = X_all + sl * NC * DIM;
+    T F[DIM][DIM], Jri[DIM][DIM], detJr;
+    kinematics_prologue<DIM, N, T>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr);
+    if (active) {
+      const T m = P.mu[cell], l = P.lambda[cell];
+      const T cs = P.cache[(size_t)cell * NQP + r];
+      T Fi[DIM][DIM], K[DIM][DIM];
+      inverse<DIM>(F, Fi);
+      grad_times<DIM>(Jri, Fi, K);
+      const T *qw = tab + 4 * N * N + N;
+      const T w = (DIM == 3) ? qw[i0] * qw[i1] * qw[i2] : qw[i0] * qw[i1];
+      const T dw = detJr * w;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) qd[(d * DIM + j) * NQP + r] = K[d][j];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const int i = voigt_first<DIM>(v), j = voigt_second<DIM>(v);
+        T s = T(0);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) s += F[i][k] * F[j][k];
+        qd[(DIM * DIM + v) * NQP + r] = (m * s - (i == j ? cs : T(0))) * dw;
+      }
+      qd[(DIM * DIM + NV) * NQP + r] = T(2) * cs * dw;
+      qd[(DIM * DIM + NV + 1) * NQP + r] = T(2) * l * dw;
+    }
+  } else {
+    if (active) {
+      const T *cp = P.cache + (size_t)cell * NQP + r;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) qd[f * NQP + r] = cp[(size_t)f * P.fs];
+    }
+  }
+  lds_barrier();
+  if (!active) return;
+  const T *S = tab, *D = tab + N * N;
+  const int ai[3] = {i0, i1, i2};
+  T acc[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) acc[c] = T(0);
+  for (int q = 0; q < NQP; ++q) {
+    const int q0 = q % N, q1 = (q / N) % N, q2 = (DIM == 3) ? q / (N * N) : 0;
+    const int qi[3] = {q0, q1, q2};
+    T gph[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      T v = T(1);
+#pragma unroll
+      for (int e = 0; e < DIM; ++e) v *= (e == d ? D : S)[qi[e] * N + ai[e]];
+      gph[d] = v;
+    }
+    T gam[DIM];
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+      T s = T(0);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) s = fma(gph[d], qd[(d * DIM + j) * NQP + q], s);
+      gam[j] = s;
+    }
+    T geo = T(0);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) geo = fma(gam[i] * gam[j], qd[(DIM * DIM + voigt_index<DIM>(i, j)) * NQP + q], geo);
+    if constexpr (VAR == V_TENSOR4) {
+      const T *M = qd + (DIM * DIM + NV) * NQP;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        T sv[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const int i = voigt_first<DIM>(v), j = voigt_second<DIM>(v);
+          sv[v] = (i == j) ? (i == c ? gam[c] : T(0)) : ((i == c ? gam[j] : T(0)) + (j == c ? gam[i] : T(0)));
+        }
+        T s = T(0);
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+          for (int w = 0; w < NV; ++w) s = fma(sv[v] * sv[w], M[sym_index<NV>(v, w) * NQP + q], s);
+        acc[c] += geo + s;
+      }
+    } else {
+      const T c1 = qd[(DIM * DIM + NV) * NQP + q], c2 = qd[(DIM * DIM + NV + 1) * NQP + q];
+      T g2 = T(0);
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) g2 = fma(gam[j], gam[j], g2);
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) acc[c] += geo + T(0.5) * c1 * (g2 + gam[c] * gam[c]) + c2 * gam[c] * gam[c];
+    }
+  }
+  const size_t node = code & kNodeMask;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    if ((code >> (kConsShift + c)) & 1u) continue;
+    T *d = P.dst + node * DIM + c;
+    if (code & kFirstBit) *d = acc[c];
+    else *d += acc[c];
+  }
+}
+
+// constrained diagonal entries are one (include/mf_nh_operator.h:1533-1534)
+template <typename T> __global__ void k_set_one(const int32_t *__restrict__ list, int n_cons, T *__restrict__ diag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_cons) diag[list[i]] = T(1);
+}
+template <typename T> __global__ void k_invert(const T *__restrict__ diag, T *__restrict__ inv, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) inv[i] = T(1) / diag[i];
+}
+template <typename T>
+__global__ void k_scale_mul(T *__restrict__ dst, const T *__restrict__ a, const T *__restrict__ b, T omega, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = omega * a[i] * b[i];
+}
+
+}  // namespace lsmf

@@ -1,0 +1,167 @@
+// lsmf_launch.hpp -- host launchers for one (dim, N) instantiation; included
+// by the lsmf_inst_*.hip units, each of which instantiates one LaunchTable.
+#pragma once
+
+#include "lsmf_error.hpp"
+#include "lsmf_kernels.hpp"
+#include "lsmf_op.hpp"
+
+namespace lsmf {
+
+template <int DIM, int N> struct Launch {
+  using CS = CellShape<DIM, N>;
+
+  template <typename T> static VmultParams<T> params(const lsmf_op *op, int k) {
+    VmultParams<T> P{};
+    P.idx = op->d_idx;
+    P.cache = static_cast<const T *>(op->d_cache);
+    P.fs = op->field_stride;
+    P.tab = static_cast<const T *>(op->d_tab);
+    P.u = static_cast<const T *>(op->d_u);
+    P.coords = op->d_coords;
+    P.mu = static_cast<const T *>(op->d_mu);
+    P.lambda = static_cast<const T *>(op->d_lambda);
+    P.cell_begin = op->colour_ptr[k];
+    P.cell_end = op->colour_ptr[k + 1];
+    return P;
+  }
+
+  static int grid_for(int n) { return (n + CS::CPW - 1) / CS::CPW; }
+
+  template <typename T, int VAR, bool ADD, bool CONSTR, unsigned PH>
+  static void run_colours(const lsmf_op *op, T *dst, const T *src, hipStream_t s) {
+    const int nc = (int)op->colour_ptr.size() - 1;
+    for (int k = 0; k < nc; ++k) {
+      VmultParams<T> P = params<T>(op, k);
+      P.src = src;
+      P.dst = dst;
+      const int n = P.cell_end - P.cell_begin;
+      if (n <= 0) continue;
+      hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, ADD, CONSTR, PH>), dim3(grid_for(n)), dim3(kWG), 0, s, P);
+    }
+  }
+
+  template <typename T, int VAR>
+  static int vmult_t(const lsmf_op *op, void *dst_, const void *src_, bool add, bool constr, unsigned ph,
+                     hipStream_t s) {
+    T *dst = static_cast<T *>(dst_);
+    const T *src = static_cast<const T *>(src_);
+    constexpr unsigned FULL = PH_SF | PH_QD;
+    if (constr) {
+      if (ph == FULL) {
+        if (add) run_colours<T, VAR, true, true, FULL>(op, dst, src, s);
+        else run_colours<T, VAR, false, true, FULL>(op, dst, src, s);
+      } else if (ph == PH_SF) {
+        run_colours<T, VAR, false, true, PH_SF>(op, dst, src, s);
+      } else {
+        run_colours<T, VAR, false, true, 0u>(op, dst, src, s);
+      }
+      if (op->n_cons > 0)
+        hipLaunchKernelGGL(k_constrained_identity<T>, dim3((op->n_cons + 255) / 256), dim3(256), 0, s, op->d_cons,
+                           op->n_cons, src, dst, add ? 1 : 0);
+    } else {
+      if (add) run_colours<T, VAR, true, false, FULL>(op, dst, src, s);
+      else run_colours<T, VAR, false, false, FULL>(op, dst, src, s);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
+  }
+
+  static int vmult(const lsmf_op *op, void *dst, const void *src, bool add, bool constr, unsigned ph,
+                   hipStream_t s) {
+    const bool f64 = op->precision == LSMF_FP64;
+    switch (op->caching) {
+      case LSMF_CACHING_TENSOR4:
+        return f64 ? vmult_t<double, V_TENSOR4>(op, dst, src, add, constr, ph, s)
+                   : vmult_t<float, V_TENSOR4>(op, dst, src, add, constr, ph, s);
+      case LSMF_CACHING_TENSOR2:
+        return f64 ? vmult_t<double, V_TENSOR2>(op, dst, src, add, constr, ph, s)
+                   : vmult_t<float, V_TENSOR2>(op, dst, src, add, constr, ph, s);
+      case LSMF_CACHING_SCALAR:
+        return f64 ? vmult_t<double, V_SCALAR>(op, dst, src, add, constr, ph, s)
+                   : vmult_t<float, V_SCALAR>(op, dst, src, add, constr, ph, s);
+      default:
+        return fail(LSMF_ERR_UNKNOWN_CACHING, "vmult: caching variant not available on the GPU");
+    }
+  }
+
+  template <typename T, int VAR> static int cache_t(const lsmf_op *op, hipStream_t s) {
+    CacheParams P{};
+    P.idx = op->d_idx;
+    P.tab = op->d_tab64;
+    P.u = op->d_u64;
+    P.coords = op->d_coords;
+    P.mu = op->d_mu64;
+    P.lambda = op->d_lambda64;
+    P.cache = op->d_cache;
+    P.fs = op->field_stride;
+    P.n_cells = op->n_cells;
+    P.bad = op->d_bad;
+    hipLaunchKernelGGL((k_cache<DIM, N, T, VAR>), dim3(grid_for(op->n_cells)), dim3(kWG), 0, s, P);
+    return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
+  }
+
+  static int cache(const lsmf_op *op, hipStream_t s) {
+    const bool f64 = op->precision == LSMF_FP64;
+    switch (op->caching) {
+      case LSMF_CACHING_TENSOR4:
+        return f64 ? cache_t<double, V_TENSOR4>(op, s) : cache_t<float, V_TENSOR4>(op, s);
+      case LSMF_CACHING_TENSOR2:
+        return f64 ? cache_t<double, V_TENSOR2>(op, s) : cache_t<float, V_TENSOR2>(op, s);
+      case LSMF_CACHING_SCALAR:
+        return f64 ? cache_t<double, V_SCALAR>(op, s) : cache_t<float, V_SCALAR>(op, s);
+      default:
+        return fail(LSMF_ERR_UNKNOWN_CACHING, "cache: caching variant not available on the GPU");
+    }
+  }
+
+  template <typename T, int VAR> static int diagonal_t(const lsmf_op *op, hipStream_t s) {
+    const int nc = (int)op->colour_ptr.size() - 1;
+    for (int k = 0; k < nc; ++k) {
+      VmultParams<T> P = params<T>(op, k);
+      P.dst = static_cast<T *>(op->d_diag);
+      const int n = P.cell_end - P.cell_begin;
+      if (n <= 0) continue;
+      hipLaunchKernelGGL((k_diagonal<DIM, N, T, VAR>), dim3(grid_for(n)), dim3(kWG), 0, s, P);
+    }
+    T *diag = static_cast<T *>(op->d_diag);
+    if (op->n_cons > 0)
+      hipLaunchKernelGGL(k_set_one<T>, dim3((op->n_cons + 255) / 256), dim3(256), 0, s, op->d_cons, op->n_cons, diag);
+    const size_t n = op->n_dofs;
+    hipLaunchKernelGGL(k_invert<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, diag,
+                       static_cast<T *>(op->d_inv_diag), n);
+    return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
+  }
+
+  static int diagonal(const lsmf_op *op, hipStream_t s) {
+    const bool f64 = op->precision == LSMF_FP64;
+    switch (op->caching) {
+      case LSMF_CACHING_TENSOR4:
+        return f64 ? diagonal_t<double, V_TENSOR4>(op, s) : diagonal_t<float, V_TENSOR4>(op, s);
+      case LSMF_CACHING_TENSOR2:
+        return f64 ? diagonal_t<double, V_TENSOR2>(op, s) : diagonal_t<float, V_TENSOR2>(op, s);
+      case LSMF_CACHING_SCALAR:
+        return f64 ? diagonal_t<double, V_SCALAR>(op, s) : diagonal_t<float, V_SCALAR>(op, s);
+      default:
+        return fail(LSMF_ERR_UNKNOWN_CACHING, "diagonal: caching variant not available on the GPU");
+    }
+  }
+
+  static int residual(const lsmf_op *op, double *rhs, hipStream_t s) {
+    if (op->precision != LSMF_FP64) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual needs an FP64 operator");
+    constexpr unsigned FULL = PH_SF | PH_QD;
+    run_colours<double, V_RESIDUAL, false, false, FULL>(op, rhs, nullptr, s);
+    return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
+  }
+
+  static const LaunchTable *table() {
+    static const LaunchTable t{&vmult, &cache, &diagonal, &residual};
+    return &t;
+  }
+};
+
+}  // namespace lsmf
+
+#define LSMF_INSTANTIATE(D, NN)                                                          \
+  namespace lsmf {                                                                       \
+  const LaunchTable *launch_table_##D##_##NN() { return Launch<D, NN>::table(); }        \
+  }

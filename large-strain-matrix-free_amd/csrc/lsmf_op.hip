@@ -1,0 +1,415 @@
+// lsmf_op.hip -- C ABI of the operator (include/lsmf.h): creation,
+// colouring, index tables, material, linearisation, cache, diagonal, vmult,
+// residual.  Host logic only; kernels live in lsmf_kernels.hpp and are
+// instantiated per (dim, N) in lsmf_inst_*.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lsmf.h"
+#include "lsmf_error.hpp"
+#include "lsmf_fe.hpp"
+#include "lsmf_kernels.hpp"
+#include "lsmf_op.hpp"
+
+namespace lsmf {
+
+std::string &last_error_string() {
+  static thread_local std::string s;
+  return s;
+}
+
+const LaunchTable *launch_table_2_2();
+const LaunchTable *launch_table_2_3();
+const LaunchTable *launch_table_2_5();
+const LaunchTable *launch_table_3_2();
+const LaunchTable *launch_table_3_3();
+const LaunchTable *launch_table_3_5();
+
+const LaunchTable *launch_table(int dim, int N) {
+  if (dim == 2 && N == 2) return launch_table_2_2();
+  if (dim == 2 && N == 3) return launch_table_2_3();
+  if (dim == 2 && N == 5) return launch_table_2_5();
+  if (dim == 3 && N == 2) return launch_table_3_2();
+  if (dim == 3 && N == 3) return launch_table_3_3();
+  if (dim == 3 && N == 5) return launch_table_3_5();
+  return nullptr;
+}
+
+size_t elem_size(const lsmf_op *op) { return op->precision == LSMF_FP64 ? sizeof(double) : sizeof(float); }
+
+int n_cache_fields(int dim, int caching) {
+  const int nv = dim == 3 ? 6 : 3;
+  switch (caching) {
+    case LSMF_CACHING_TENSOR4: return dim * dim + nv + nv * (nv + 1) / 2;
+    case LSMF_CACHING_TENSOR2: return dim * dim + nv + 2;
+    case LSMF_CACHING_SCALAR: return 1;
+    default: return -1;
+  }
+}
+
+namespace {
+
+#define LSMF_HIP_TRY(expr, what)                                                                   \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return fail(e_ == hipErrorOutOfMemory ? LSMF_ERR_OUT_OF_MEMORY : LSMF_ERR_HIP,               \
+                  std::string(what) + ": " + hipGetErrorString(e_));                               \
+  } while (0)
+
+template <typename T> int dalloc(T **p, size_t n, const char *what) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(e == hipErrorOutOfMemory ? LSMF_ERR_OUT_OF_MEMORY : LSMF_ERR_HIP,
+                std::string("hipMalloc(") + what + "): " + hipGetErrorString(e));
+  }
+  return 0;
+}
+
+void free_op(lsmf_op *op) {
+  void *ptrs[] = {op->d_idx,  op->d_coords,    op->d_tab,  op->d_tab64, op->d_mu64,     op->d_lambda64,
+                  op->d_mu,   op->d_lambda,    op->d_cache, op->d_diag, op->d_inv_diag, op->d_cons,
+                  op->d_bad};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (op->d_u64) (void)hipFree(op->d_u64);
+  if (op->d_u && op->d_u != op->d_u64) (void)hipFree(op->d_u);
+}
+
+__global__ void k_zero_int(int *p) { *p = 0; }
+
+__global__ void k_convert_f64_f32(const double *__restrict__ a, float *__restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = float(a[i]);
+}
+
+}  // namespace
+}  // namespace lsmf
+
+using namespace lsmf;
+
+extern "C" {
+
+const char *lsmf_last_error(void) { return last_error_string().c_str(); }
+
+lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *opt, lsmf_op **out) {
+  if (!mesh || !opt || !out) return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: null argument");
+  *out = nullptr;
+  const int dim = mesh->dim, p = mesh->degree, nq = opt->n_q_1d;
+  if (dim != 2 && dim != 3) return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: dim must be 2 or 3");
+  if (opt->precision != LSMF_FP64 && opt->precision != LSMF_FP32)
+    return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: unknown precision");
+  if (n_cache_fields(dim, opt->caching) < 0)
+    return fail(LSMF_ERR_UNKNOWN_CACHING, "Unknown caching");
+  if (nq != p + 1 || !launch_table(dim, nq))
+    return fail(LSMF_ERR_UNSUPPORTED, "lsmf_op_create: (dim, degree, n_q_1d) = (" + std::to_string(dim) + ", " +
+                                          std::to_string(p) + ", " + std::to_string(nq) + ") is not instantiated");
+  if (mesh->n_cells <= 0 || mesh->n_nodes <= 0 || !mesh->cell_nodes || !mesh->coords)
+    return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: empty mesh");
+  if (mesh->n_nodes >= (1 << 28)) return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: more than 2^28 nodes");
+  const int N = nq;
+  int nqp = 1;
+  for (int d = 0; d < dim; ++d) nqp *= N;
+  const int nc = mesh->n_cells;
+  for (long long i = 0; i < (long long)nc * nqp; ++i)
+    if (mesh->cell_nodes[i] < 0 || mesh->cell_nodes[i] >= mesh->n_nodes)
+      return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: cell_nodes entry out of range");
+
+  lsmf_op *op = new (std::nothrow) lsmf_op;
+  if (!op) return fail(LSMF_ERR_OUT_OF_MEMORY, "lsmf_op_create: host allocation");
+  op->dim = dim;
+  op->degree = p;
+  op->nq = nq;
+  op->N = N;
+  op->nqp = nqp;
+  op->precision = opt->precision;
+  op->caching = opt->caching;
+  op->n_cells = nc;
+  op->n_nodes = mesh->n_nodes;
+  op->n_dofs = (size_t)mesh->n_nodes * dim;
+  op->n_fields = n_cache_fields(dim, opt->caching);
+  op->field_stride = (size_t)nc * nqp;
+
+  // --- colouring: given or greedy first fit
+  std::vector<int> colour(nc, 0);
+  if (mesh->cell_colour) {
+    for (int c = 0; c < nc; ++c) {
+      colour[c] = mesh->cell_colour[c];
+      if (colour[c] < 0 || colour[c] > 63) {
+        delete op;
+        return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: colour out of [0, 63]");
+      }
+    }
+  } else {
+    std::vector<uint64_t> used(mesh->n_nodes, 0);
+    for (int c = 0; c < nc; ++c) {
+      uint64_t m = 0;
+      for (int a = 0; a < nqp; ++a) m |= used[mesh->cell_nodes[(size_t)c * nqp + a]];
+      int col = 0;
+      while (col < 64 && (m >> col) & 1ull) ++col;
+      if (col == 64) {
+        delete op;
+        return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: greedy colouring needs more than 64 colours");
+      }
+      colour[c] = col;
+      for (int a = 0; a < nqp; ++a) used[mesh->cell_nodes[(size_t)c * nqp + a]] |= (1ull << col);
+    }
+  }
+  int n_col = 0;
+  for (int c = 0; c < nc; ++c) n_col = std::max(n_col, colour[c] + 1);
+  op->colour_ptr.assign(n_col + 1, 0);
+  for (int c = 0; c < nc; ++c) op->colour_ptr[colour[c] + 1]++;
+  for (int k = 0; k < n_col; ++k) op->colour_ptr[k + 1] += op->colour_ptr[k];
+  op->perm.assign(nc, 0);
+  op->inv_perm.assign(nc, 0);
+  {
+    std::vector<int> fill(op->colour_ptr.begin(), op->colour_ptr.end() - 1);
+    for (int c = 0; c < nc; ++c) {
+      const int pos = fill[colour[c]]++;
+      op->perm[pos] = c;
+      op->inv_perm[c] = pos;
+    }
+  }
+  // colour conflict check: no node twice inside one colour; min colour per node
+  std::vector<int> stamp(mesh->n_nodes, -1), min_col(mesh->n_nodes, 1 << 30);
+  for (int k = 0; k < n_col; ++k)
+    for (int pos = op->colour_ptr[k]; pos < op->colour_ptr[k + 1]; ++pos) {
+      const int c = op->perm[pos];
+      for (int a = 0; a < nqp; ++a) {
+        const int nd = mesh->cell_nodes[(size_t)c * nqp + a];
+        if (stamp[nd] == k) {
+          delete op;
+          return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: two cells of one colour share a node");
+        }
+        stamp[nd] = k;
+        min_col[nd] = std::min(min_col[nd], k);
+      }
+    }
+  // --- index table (sorted cells): node | constraint bits | first-touch bit
+  std::vector<uint32_t> idx((size_t)nc * nqp);
+  for (int pos = 0; pos < nc; ++pos) {
+    const int c = op->perm[pos];
+    for (int a = 0; a < nqp; ++a) {
+      const int nd = mesh->cell_nodes[(size_t)c * nqp + a];
+      uint32_t code = (uint32_t)nd;
+      if (mesh->constrained)
+        for (int comp = 0; comp < dim; ++comp)
+          if (mesh->constrained[(size_t)nd * dim + comp]) code |= 1u << (kConsShift + comp);
+      if (colour[c] == min_col[nd]) code |= kFirstBit;
+      idx[(size_t)pos * nqp + a] = code;
+    }
+  }
+  std::vector<int32_t> cons;
+  if (mesh->constrained)
+    for (size_t i = 0; i < op->n_dofs; ++i)
+      if (mesh->constrained[i]) cons.push_back((int32_t)i);
+  op->n_cons = (int)cons.size();
+
+  // --- 1D tables: S, D, St, Dt, qp, qw
+  Tables1D t1;
+  t1.init(p, nq);
+  std::vector<double> tab(4 * N * N + 2 * N);
+  for (int q = 0; q < N; ++q)
+    for (int a = 0; a < N; ++a) {
+      tab[q * N + a] = t1.S[q * N + a];
+      tab[N * N + q * N + a] = t1.D[q * N + a];
+      tab[2 * N * N + a * N + q] = t1.S[q * N + a];
+      tab[3 * N * N + a * N + q] = t1.D[q * N + a];
+    }
+  for (int q = 0; q < N; ++q) {
+    tab[4 * N * N + q] = t1.qpts[q];
+    tab[4 * N * N + N + q] = t1.qw[q];
+  }
+
+  const size_t es = elem_size(op);
+  int rc = 0;
+  if ((rc = dalloc(&op->d_idx, idx.size(), "index table")) ||
+      (rc = dalloc(&op->d_coords, (size_t)mesh->n_nodes * dim, "coordinates")) ||
+      (rc = dalloc(&op->d_tab64, tab.size(), "tables")) ||
+      (rc = dalloc((char **)&op->d_tab, tab.size() * es, "tables")) ||
+      (rc = dalloc(&op->d_mu64, (size_t)nc, "mu")) || (rc = dalloc(&op->d_lambda64, (size_t)nc, "lambda")) ||
+      (rc = dalloc((char **)&op->d_mu, (size_t)nc * es, "mu")) ||
+      (rc = dalloc((char **)&op->d_lambda, (size_t)nc * es, "lambda")) ||
+      (rc = dalloc(&op->d_u64, op->n_dofs, "displacement")) ||
+      (rc = dalloc((char **)&op->d_cache, (size_t)op->n_fields * op->field_stride * es, "cache")) ||
+      (rc = dalloc((char **)&op->d_diag, op->n_dofs * es, "diagonal")) ||
+      (rc = dalloc((char **)&op->d_inv_diag, op->n_dofs * es, "inverse diagonal")) ||
+      (rc = dalloc(&op->d_cons, cons.size(), "constrained list")) || (rc = dalloc(&op->d_bad, 1, "counter"))) {
+    free_op(op);
+    delete op;
+    return rc;
+  }
+  if (op->precision == LSMF_FP64) op->d_u = op->d_u64;
+  else if ((rc = dalloc((char **)&op->d_u, op->n_dofs * es, "displacement (fp32)"))) {
+    free_op(op);
+    delete op;
+    return rc;
+  }
+  op->bytes_idx = idx.size() * sizeof(uint32_t);
+  op->bytes_cache = (size_t)op->n_fields * op->field_stride * es;
+  hipError_t e = hipSuccess;
+  e = hipMemcpy(op->d_idx, idx.data(), idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(op->d_coords, mesh->coords, (size_t)mesh->n_nodes * dim * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(op->d_tab64, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    if (op->precision == LSMF_FP64) {
+      e = hipMemcpy(op->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    } else {
+      std::vector<float> tf(tab.begin(), tab.end());
+      e = hipMemcpy(op->d_tab, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice);
+    }
+  }
+  if (e == hipSuccess && !cons.empty())
+    e = hipMemcpy(op->d_cons, cons.data(), cons.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    free_op(op);
+    delete op;
+    return fail(LSMF_ERR_HIP, std::string("lsmf_op_create upload: ") + hipGetErrorString(e));
+  }
+  *out = op;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_op_destroy(lsmf_op *op) {
+  if (!op) return LSMF_OK;
+  free_op(op);
+  delete op;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_set_material_per_cell(lsmf_op *op, const double *mu, const double *lambda) {
+  if (!op || !mu || !lambda) return fail(LSMF_ERR_INVALID_ARGUMENT, "set_material: null argument");
+  const int nc = op->n_cells;
+  std::vector<double> m(nc), l(nc);
+  for (int c = 0; c < nc; ++c) {
+    m[op->inv_perm[c]] = mu[c];
+    l[op->inv_perm[c]] = lambda[c];
+  }
+  LSMF_HIP_TRY(hipMemcpy(op->d_mu64, m.data(), nc * sizeof(double), hipMemcpyHostToDevice), "set_material");
+  LSMF_HIP_TRY(hipMemcpy(op->d_lambda64, l.data(), nc * sizeof(double), hipMemcpyHostToDevice), "set_material");
+  if (op->precision == LSMF_FP64) {
+    LSMF_HIP_TRY(hipMemcpy(op->d_mu, m.data(), nc * sizeof(double), hipMemcpyHostToDevice), "set_material");
+    LSMF_HIP_TRY(hipMemcpy(op->d_lambda, l.data(), nc * sizeof(double), hipMemcpyHostToDevice), "set_material");
+  } else {
+    std::vector<float> mf(m.begin(), m.end()), lf(l.begin(), l.end());
+    LSMF_HIP_TRY(hipMemcpy(op->d_mu, mf.data(), nc * sizeof(float), hipMemcpyHostToDevice), "set_material");
+    LSMF_HIP_TRY(hipMemcpy(op->d_lambda, lf.data(), nc * sizeof(float), hipMemcpyHostToDevice), "set_material");
+  }
+  op->material_set = true;
+  op->cached = false;
+  op->diag_ok = false;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_set_linearization(lsmf_op *op, const double *u_dev, lsmf_stream stream) {
+  if (!op || !u_dev) return fail(LSMF_ERR_INVALID_ARGUMENT, "set_linearization: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  LSMF_HIP_TRY(hipMemcpyAsync(op->d_u64, u_dev, op->n_dofs * sizeof(double), hipMemcpyDeviceToDevice, s),
+               "set_linearization");
+  if (op->precision == LSMF_FP32) {
+    hipLaunchKernelGGL(k_convert_f64_f32, dim3((unsigned)((op->n_dofs + 255) / 256)), dim3(256), 0, s, op->d_u64,
+                       (float *)op->d_u, op->n_dofs);
+    LSMF_HIP_TRY(hipGetLastError(), "set_linearization convert");
+  }
+  op->linearization_set = true;
+  op->cached = false;
+  op->diag_ok = false;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_cache(lsmf_op *op, lsmf_stream stream) {
+  if (!op) return fail(LSMF_ERR_INVALID_ARGUMENT, "cache: null operator");
+  if (!op->material_set) return fail(LSMF_ERR_INVALID_ARGUMENT, "cache: set_material was not called");
+  if (!op->linearization_set) return fail(LSMF_ERR_INVALID_ARGUMENT, "cache: set_linearization was not called");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, op->d_bad);
+  int rc = launch_table(op->dim, op->N)->cache(op, s);
+  if (rc) return fail(rc, "cache: kernel launch failed");
+  int bad = 0;
+  LSMF_HIP_TRY(hipMemcpyAsync(&bad, op->d_bad, sizeof(int), hipMemcpyDeviceToHost, s), "cache: read det F count");
+  LSMF_HIP_TRY(hipStreamSynchronize(s), "cache: synchronize");
+  if (bad > 0) {
+    op->cached = false;
+    return fail(LSMF_ERR_NONPOSITIVE_DET_F,
+                "det_F is not positive at " + std::to_string(bad) + " quadrature points");
+  }
+  op->cached = true;
+  op->diag_ok = false;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_compute_diagonal(lsmf_op *op, lsmf_stream stream) {
+  if (!op) return fail(LSMF_ERR_INVALID_ARGUMENT, "compute_diagonal: null operator");
+  if (!op->cached) return fail(LSMF_ERR_NOT_CACHED, "compute_diagonal: call cache() first");
+  int rc = launch_table(op->dim, op->N)->diagonal(op, (hipStream_t)stream);
+  if (rc) return fail(rc, "compute_diagonal: kernel launch failed");
+  op->diag_ok = true;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_vmult(lsmf_op *op, void *dst, const void *src, int add, unsigned mask, lsmf_stream stream) {
+  if (!op || !dst || !src) return fail(LSMF_ERR_INVALID_ARGUMENT, "vmult: null argument");
+  if (dst == src) return fail(LSMF_ERR_INVALID_ARGUMENT, "vmult: dst and src alias");
+  if (!op->cached) return fail(LSMF_ERR_NOT_CACHED, "vmult: call cache() first");
+  const bool plain = (mask & LSMF_MASK_PLAIN) != 0;
+  unsigned ph = mask & (LSMF_MASK_SF | LSMF_MASK_QD);
+  if (!(ph & LSMF_MASK_SF)) ph = 0;  // no sum factorization: gather/scatter only
+  if (plain || add) ph = LSMF_MASK_SF | LSMF_MASK_QD;
+  const int rc = launch_table(op->dim, op->N)->vmult(op, dst, src, add != 0, !plain, ph, (hipStream_t)stream);
+  if (rc) return fail(rc, "vmult: kernel launch failed");
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_get_diagonal(lsmf_op *op, const void **diag_dev, const void **inv_diag_dev) {
+  if (!op) return fail(LSMF_ERR_INVALID_ARGUMENT, "get_diagonal: null operator");
+  if (!op->diag_ok) return fail(LSMF_ERR_NOT_CACHED, "get_diagonal: call compute_diagonal() first");
+  if (diag_dev) *diag_dev = op->d_diag;
+  if (inv_diag_dev) *inv_diag_dev = op->d_inv_diag;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_precondition_jacobi(lsmf_op *op, void *dst, const void *src, double omega, lsmf_stream stream) {
+  if (!op || !dst || !src) return fail(LSMF_ERR_INVALID_ARGUMENT, "precondition_Jacobi: null argument");
+  if (!op->diag_ok) return fail(LSMF_ERR_NOT_CACHED, "precondition_Jacobi: call compute_diagonal() first");
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned g = (unsigned)((op->n_dofs + 255) / 256);
+  if (op->precision == LSMF_FP64)
+    hipLaunchKernelGGL(k_scale_mul<double>, dim3(g), dim3(256), 0, s, (double *)dst, (const double *)src,
+                       (const double *)op->d_inv_diag, omega, op->n_dofs);
+  else
+    hipLaunchKernelGGL(k_scale_mul<float>, dim3(g), dim3(256), 0, s, (float *)dst, (const float *)src,
+                       (const float *)op->d_inv_diag, (float)omega, op->n_dofs);
+  LSMF_HIP_TRY(hipGetLastError(), "precondition_Jacobi");
+  return LSMF_OK;
+}
+
+size_t lsmf_op_size(const lsmf_op *op) { return op ? op->n_dofs : 0; }
+
+size_t lsmf_memory_consumption(const lsmf_op *op) {
+  if (!op) return 0;
+  const size_t es = elem_size(op);
+  size_t b = op->bytes_cache + op->bytes_idx + 2 * (size_t)op->n_cells * es;
+  if (op->caching == LSMF_CACHING_SCALAR) b += op->n_dofs * es + (size_t)op->n_nodes * op->dim * sizeof(double);
+  return b;
+}
+
+lsmf_status lsmf_residual(lsmf_op *op, double *rhs_dev, lsmf_stream stream) {
+  if (!op || !rhs_dev) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual: null argument");
+  if (op->precision != LSMF_FP64) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual: operator must be FP64");
+  if (!op->material_set || !op->linearization_set)
+    return fail(LSMF_ERR_INVALID_ARGUMENT, "residual: material and linearisation must be set");
+  const int rc = launch_table(op->dim, op->N)->residual(op, rhs_dev, (hipStream_t)stream);
+  if (rc) return fail(rc, "residual: kernel launch failed");
+  return LSMF_OK;
+}
+
+}  // extern "C"

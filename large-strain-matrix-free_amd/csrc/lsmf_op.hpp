@@ -1,0 +1,58 @@
+// lsmf_op.hpp -- the operator object behind the opaque lsmf_op handle, and
+// the per-(dim, N) launch table the instantiation units fill in.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lsmf.h"
+
+struct lsmf_op {
+  // shape
+  int dim = 0, degree = 0, nq = 0, N = 0, nqp = 0;
+  int precision = LSMF_FP64, caching = LSMF_CACHING_TENSOR4;
+  int n_cells = 0, n_nodes = 0;
+  size_t n_dofs = 0;
+  int n_fields = 0;
+  size_t field_stride = 0;
+  // colouring (host): sorted cells [colour_ptr[k], colour_ptr[k+1])
+  std::vector<int> colour_ptr;
+  std::vector<int> perm;      // sorted position -> mesh cell
+  std::vector<int> inv_perm;  // mesh cell -> sorted position
+  // device data
+  uint32_t *d_idx = nullptr;        // [n_cells][nqp] node | constraint bits | first-touch bit
+  double *d_coords = nullptr;       // [n_nodes][dim]
+  void *d_tab = nullptr;            // S, D, St, Dt, qp, qw in operator precision
+  double *d_tab64 = nullptr;        // same tables in FP64 (cache kernel)
+  double *d_mu64 = nullptr, *d_lambda64 = nullptr;  // per sorted cell
+  void *d_mu = nullptr, *d_lambda = nullptr;        // operator precision
+  double *d_u64 = nullptr;          // linearisation state copy
+  void *d_u = nullptr;              // operator precision copy (scalar variant)
+  void *d_cache = nullptr;
+  void *d_diag = nullptr, *d_inv_diag = nullptr;
+  int32_t *d_cons = nullptr;        // constrained DoF list
+  int n_cons = 0;
+  int *d_bad = nullptr;
+  bool material_set = false, linearization_set = false, cached = false, diag_ok = false;
+  size_t bytes_idx = 0, bytes_cache = 0;
+};
+
+namespace lsmf {
+
+size_t elem_size(const lsmf_op *op);
+
+// launch entry points of one (dim, N) instantiation unit
+struct LaunchTable {
+  int (*vmult)(const lsmf_op *, void *dst, const void *src, bool add, bool constr, unsigned ph, hipStream_t);
+  int (*cache)(const lsmf_op *, hipStream_t);
+  int (*diagonal)(const lsmf_op *, hipStream_t);
+  int (*residual)(const lsmf_op *, double *rhs, hipStream_t);
+};
+
+const LaunchTable *launch_table(int dim, int N);
+int n_cache_fields(int dim, int caching);
+
+}  // namespace lsmf
