@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark: matrix-free Neo-Hookean tangent vmult on one MI355X.
+
+Workload (default): the precision / large-size configuration -- unit cube,
+128^3 hexes, Q2 elements (3^3 Gauss points), 50,923,779 DoFs, nu = 0.45 with
+200 seeded ball inclusions (r = 0.03, 10x stiffness), tensor4 caching, FP64,
+x = 0 clamped and u_x prescribed at x = 1, linearised about the seeded smooth
+field u_i = 0.02 sin(pi k_i . x).  One step = one operator application
+y = A x over the whole mesh (the reference's vmult, timed the same way as its
+n_times = 10 loop, include/mf_elasticity.h:2354-2448), with x uniform in
+[0,1) and zero on constrained DoFs.  Inputs are resident in HBM before the
+timed region.
+
+Prints one JSON line (stdout) with the throughput in GDoF/s, the HBM roofline
+of the fused cell kernel (algorithmic bytes of the reference's memory model,
+doc/large_strain.tex:739-753, divided by the kernel's event-timed duration)
+and the CPU oracle timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "large-strain-matrix-free_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Q2-3D tangent vmult GDoF/s; Newton-solve s, FP32-GMG vs FP64, same tol"
+
+
+def progress(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(dim, caching, precision, n_qp, n_nodes):
+    """memory model of the reference (doc/large_strain.tex:739-753):
+    vector access per node + physics and metric terms per quadrature point"""
+    es = 8 if precision == "fp64" else 4
+    nv = dim * (dim + 1) // 2
+    if caching == "scalar":
+        per_qp = 1 + 2 * dim * dim + 1
+        per_node = 3 * dim
+    elif caching == "tensor2":
+        per_qp = 2 + nv + dim * dim + 1
+        per_node = 2 * dim
+    else:
+        per_qp = nv + nv * nv + dim * dim + 1
+        per_node = 2 * dim
+    return es * (per_qp * n_qp + per_node * n_nodes)
+
+
+def cpu_baseline(cfg_id, caching, n_sample, seconds):
+    """FP64 oracle (OpenMP, sum factorization) on a smaller cube of the same
+    configuration; GDoF/s of its vmult."""
+    import numpy as np
+    import lsmf
+    import oracle
+
+    mesh, mu, lam = lsmf.build_config_mesh(cfg_id, n=n_sample)
+    u, _ = lsmf.smooth_displacement(mesh.coords, 0.02, lsmf.workload.SEED + cfg_id)
+    cons = mesh.constrained_mask({0: list(range(mesh.dim)), 1: [0]})
+    op = oracle.OracleOperator(mesh.dim, mesh.degree, mesh.cell_nodes, mesh.coords, cons, caching=caching)
+    op.set_material(mu, lam)
+    op.set_linearization(u)
+    op.cache()
+    x = np.random.default_rng(0).random(mesh.n_dofs)
+    x[cons == 1] = 0.0
+    op.vmult(x)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        op.vmult(x)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 1000:
+            break
+    return {
+        "value": mesh.n_dofs * n / el / 1e9,
+        "unit": "GDoF/s",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": (f"{n_sample}^3-cell Q2 {caching} FP64 vmult of the same configuration "
+                   f"({mesh.n_dofs} DoF), {n} vmults in {el:.1f} s, oracle C++ (OpenMP, sum factorization)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--n", type=int, default=None, help="override cells per edge")
+    ap.add_argument("--caching", default=None)
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-n", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on exactly one GPU: the operator is measured on one MI355X "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import lsmf
+
+    cfg = lsmf.CONFIGS[args.config]
+    caching = args.caching or cfg["caching"]
+    progress(f"building config {args.config} mesh")
+    mesh, mu, lam = lsmf.build_config_mesh(args.config, n=args.n)
+    u, _ = lsmf.smooth_displacement(mesh.coords, 0.02, lsmf.workload.SEED + args.config)
+    cons = mesh.constrained_mask({0: list(range(mesh.dim)), 1: [0]})
+    dev = torch.device("cuda:0")
+    progress(f"creating operator: {mesh.n_cells} cells, {mesh.n_dofs} DoF, {caching} {args.precision}")
+    op = lsmf.NeoHookOperator(mesh, precision=args.precision, caching=caching, constrained=cons)
+    op.set_material(mu, lam)
+    ud = torch.tensor(u, device=dev)
+    del u
+    op.set_linearization(ud)
+    op.cache()
+    dt = torch.float64 if args.precision == "fp64" else torch.float32
+    g = torch.Generator(device=dev)
+    g.manual_seed(2354)
+    x = torch.rand(mesh.n_dofs, dtype=dt, device=dev, generator=g)
+    x[torch.tensor(cons, device=dev).bool()] = 0
+    y = torch.empty_like(x)
+    progress("warmup")
+    for _ in range(args.warmup):
+        op.vmult(y, x)
+    torch.cuda.synchronize()
+
+    progress(f"timing {args.steps} vmults")
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    op.profile_enable(True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        op.vmult(y, x)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    total_ms = e0.elapsed_time(e1)
+    kern_ms, n_launch = op.profile_read()
+    op.profile_enable(False)
+    ms_per_step = total_ms / args.steps
+
+    nq = mesh.degree + 1
+    n_qp = mesh.n_cells * nq ** mesh.dim
+    alg_bytes = algorithmic_bytes(mesh.dim, caching, args.precision, n_qp, mesh.n_nodes)
+    launches_per_vmult = n_launch / args.steps
+    bytes_per_launch = alg_bytes / launches_per_vmult
+    avg_launch_s = kern_ms / n_launch / 1e3
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_config{args.config}_{caching}_{args.precision}.json")
+    if args.n is None and os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        progress(f"CPU baseline: oracle on {args.cpu_n}^3 sample for ~{args.cpu_seconds:.0f} s")
+        del op
+        torch.cuda.empty_cache()
+        cpu = cpu_baseline(args.config, caching, args.cpu_n, args.cpu_seconds)
+
+    workload = (f"config {args.config}: {mesh.n_cells_dir[0]}^{mesh.dim} Q{mesh.degree} {caching} "
+                f"{args.precision} vmult, {mesh.n_dofs} DoF")
+    line = {
+        "metric": METRIC,
+        "value": round(mesh.n_dofs / (ms_per_step / 1e3) / 1e9, 4),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": workload, "dim": mesh.dim, "degree": mesh.degree, "n_q_1d": nq,
+                   "cells": mesh.n_cells, "dofs": mesh.n_dofs, "qp": n_qp, "caching": caching,
+                   "precision": args.precision, "nu": cfg["nu"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_vmult (fused gather/SF/tangent/integrate/scatter, one launch per colour)",
+                     "kernel_avg_ms": round(avg_launch_s * 1e3, 5), "launches_per_step": launches_per_vmult,
+                     "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

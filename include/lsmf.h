@@ -171,6 +171,16 @@ lsmf_status lsmf_residual(lsmf_op *op, double *rhs_dev, lsmf_stream stream);
 /* thread-local description of the last error on this thread */
 const char *lsmf_last_error(void);
 
+/* Per-launch timing of the operator's fused cell kernels with HIP events on
+ * the launch stream, the counterpart of the reference's TimerOutput and LIKWID
+ * regions around vmult (include/mf_nh_operator.h:108-119,
+ * include/mf_elasticity.h:2354-2448).  enable != 0 starts recording (and
+ * discards earlier records); lsmf_profile_read synchronises the device,
+ * returns the summed kernel time in milliseconds and the number of launches
+ * since enabling, and resets the counters. */
+lsmf_status lsmf_profile_enable(lsmf_op *op, int enable);
+lsmf_status lsmf_profile_read(lsmf_op *op, double *total_ms, int *n_launches);
+
 /* ------------------------------------------------------------------------
  * Structured meshes of the unit square / cube (and the Cook membrane used by
  * the reference's tests).  Replaces GridGenerator::subdivided_hyper_rectangle

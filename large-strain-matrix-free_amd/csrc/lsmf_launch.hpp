@@ -37,7 +37,9 @@ template <int DIM, int N> struct Launch {
       P.dst = dst;
       const int n = P.cell_end - P.cell_begin;
       if (n <= 0) continue;
+      if (op->prof) op->prof->before(s);
       hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, ADD, CONSTR, PH>), dim3(grid_for(n)), dim3(kWG), 0, s, P);
+      if (op->prof) op->prof->after(s);
     }
   }
 

@@ -281,6 +281,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
 
 lsmf_status lsmf_op_destroy(lsmf_op *op) {
   if (!op) return LSMF_OK;
+  delete op->prof;
   free_op(op);
   delete op;
   return LSMF_OK;
@@ -400,6 +401,33 @@ size_t lsmf_memory_consumption(const lsmf_op *op) {
   size_t b = op->bytes_cache + op->bytes_idx + 2 * (size_t)op->n_cells * es;
   if (op->caching == LSMF_CACHING_SCALAR) b += op->n_dofs * es + (size_t)op->n_nodes * op->dim * sizeof(double);
   return b;
+}
+
+lsmf_status lsmf_profile_enable(lsmf_op *op, int enable) {
+  if (!op) return fail(LSMF_ERR_INVALID_ARGUMENT, "profile_enable: null operator");
+  if (!op->prof) op->prof = new (std::nothrow) Profiler;
+  if (!op->prof) return fail(LSMF_ERR_OUT_OF_MEMORY, "profile_enable: host allocation");
+  op->prof->on = enable != 0;
+  op->prof->used = 0;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_profile_read(lsmf_op *op, double *total_ms, int *n_launches) {
+  if (!op || !total_ms || !n_launches) return fail(LSMF_ERR_INVALID_ARGUMENT, "profile_read: null argument");
+  *total_ms = 0.0;
+  *n_launches = 0;
+  if (!op->prof) return LSMF_OK;
+  LSMF_HIP_TRY(hipDeviceSynchronize(), "profile_read: synchronize");
+  double sum = 0.0;
+  for (size_t i = 0; i < op->prof->used; ++i) {
+    float ms = 0.f;
+    LSMF_HIP_TRY(hipEventElapsedTime(&ms, op->prof->start[i], op->prof->stop[i]), "profile_read: elapsed");
+    sum += ms;
+  }
+  *total_ms = sum;
+  *n_launches = (int)op->prof->used;
+  op->prof->used = 0;
+  return LSMF_OK;
 }
 
 lsmf_status lsmf_residual(lsmf_op *op, double *rhs_dev, lsmf_stream stream) {

@@ -10,7 +10,41 @@
 
 #include "lsmf.h"
 
+namespace lsmf {
+// per-launch event timing of the operator's cell kernels (the reference's
+// TimerOutput / LIKWID regions around vmult, include/mf_nh_operator.h:108-119)
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> start, stop;
+  size_t used = 0;
+  void before(hipStream_t s) {
+    if (!on) return;
+    if (used == start.size()) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess) return;
+      if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return;
+      }
+      start.push_back(a);
+      stop.push_back(b);
+    }
+    (void)hipEventRecord(start[used], s);
+  }
+  void after(hipStream_t s) {
+    if (!on || used >= start.size()) return;
+    (void)hipEventRecord(stop[used], s);
+    ++used;
+  }
+  ~Profiler() {
+    for (auto e : start) (void)hipEventDestroy(e);
+    for (auto e : stop) (void)hipEventDestroy(e);
+  }
+};
+}  // namespace lsmf
+
 struct lsmf_op {
+  lsmf::Profiler *prof = nullptr;
   // shape
   int dim = 0, degree = 0, nq = 0, N = 0, nqp = 0;
   int precision = LSMF_FP64, caching = LSMF_CACHING_TENSOR4;

@@ -22,6 +22,16 @@ import ctypes
 import os
 import numpy as np
 
+# PyTorch-ROCm ships its own libamdhip64.so.7 / libhsa-runtime64.so.1 (same
+# sonames as /opt/rocm).  Loading torch first makes liblsmf.so bind to those
+# already-loaded objects, so the process holds a single HIP runtime; loading
+# liblsmf.so first would make torch load a second one, and the two do not
+# share devices.
+try:
+    import torch  # noqa: F401
+except ImportError:  # CPU-only host logic still works without torch
+    pass
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIBPATH = os.path.join(_HERE, "liblsmf.so")
 
@@ -84,6 +94,8 @@ _sig = {
     "lsmf_memory_consumption": (ctypes.c_size_t, [_vp]),
     "lsmf_residual": (_i, [_vp, _vp, _vp]),
     "lsmf_last_error": (ctypes.c_char_p, []),
+    "lsmf_profile_enable": (_i, [_vp, _i]),
+    "lsmf_profile_read": (_i, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
     "lsmf_mesh_create": (_i, [_i, _i, _vp, _vp, _vp, _i, ctypes.c_double, ctypes.POINTER(ctypes.POINTER(_Mesh))]),
     "lsmf_mesh_destroy": (_i, [ctypes.POINTER(_Mesh)]),
     "lsmf_mesh_boundary_faces": (_i, [ctypes.POINTER(_Mesh), _i, _vp, _vp]),
@@ -281,6 +293,16 @@ class NeoHookOperator:
 
     def precondition_Jacobi(self, dst, src, omega, stream=None):
         _check(_lib.lsmf_precondition_jacobi(self._h, _ptr(dst), _ptr(src), float(omega), _stream(stream)))
+
+    def profile_enable(self, enable=True):
+        _check(_lib.lsmf_profile_enable(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        """(summed kernel milliseconds, number of cell-kernel launches) since enable"""
+        ms = ctypes.c_double()
+        n = ctypes.c_int()
+        _check(_lib.lsmf_profile_read(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def residual(self, rhs, stream=None):
         import torch

@@ -1,0 +1,218 @@
+"""GPU parity of the HIP operator against the FP64 oracle, through the C ABI.
+
+Tolerances: FP64 vmult / diagonal / residual within 1e-12 relative (max-norm,
+the BASELINE correctness target for vmult); FP32 level operators within
+2e-5 relative (the cache is rounded to FP32 after FP64 kinematics).
+Full-size configurations are checked through size-independent properties
+(symmetry, null space, agreement between caching variants, add = 2x set).
+"""
+import numpy as np
+import pytest
+
+import lsmf
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def problem(dim, p, n, seed=5, amplitude=0.02, n_incl=4):
+    mesh = lsmf.Mesh(dim, p, [n] * dim)
+    mu, lam, _ = lsmf.inclusion_material(mesh.cell_centres(), dim, 1.0, 0.3, n_incl, 0.25, 10.0, seed)
+    u, _ = lsmf.smooth_displacement(mesh.coords, amplitude, seed)
+    cons = mesh.constrained_mask({0: list(range(dim)), 1: [0]})
+    return mesh, mu, lam, u, cons
+
+
+def gpu_op(mesh, mu, lam, u, cons, caching, precision="fp64"):
+    import torch
+    op = lsmf.NeoHookOperator(mesh, precision=precision, caching=caching, constrained=cons)
+    op.set_material(mu, lam)
+    op.set_linearization(torch.tensor(u, dtype=torch.float64, device="cuda"))
+    op.cache()
+    return op
+
+
+def orc_op(mesh, mu, lam, u, cons, caching="tensor4"):
+    op = oracle.OracleOperator(mesh.dim, mesh.degree, mesh.cell_nodes, mesh.coords, cons, caching=caching)
+    op.set_material(mu, lam)
+    op.set_linearization(u)
+    op.cache()
+    return op
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+CASES = [(3, 2, 4), (3, 2, 5), (2, 1, 6), (2, 2, 5), (3, 1, 4), (3, 4, 2), (2, 4, 3)]
+
+
+@pytest.mark.parametrize("dim,p,n", CASES, ids=[f"{d}d-Q{p}-n{n}" for d, p, n in CASES])
+@pytest.mark.parametrize("caching", ["scalar", "tensor2", "tensor4"])
+def test_vmult_fp64_matches_oracle(dim, p, n, caching, device):
+    import torch
+    log(f"[vmult fp64] dim={dim} p={p} n={n} caching={caching}")
+    mesh, mu, lam, u, cons = problem(dim, p, n)
+    ref = orc_op(mesh, mu, lam, u, cons, caching)
+    op = gpu_op(mesh, mu, lam, u, cons, caching)
+    v = np.random.default_rng(1).random(mesh.n_dofs)
+    src = torch.tensor(v, device=device)
+    dst = torch.full_like(src, np.nan)
+    op.vmult(dst, src)
+    expect = ref.vmult(v)
+    assert rel(dst.cpu().numpy(), expect) < 1e-12
+    # vmult_add: dst += A src
+    w = np.random.default_rng(2).random(mesh.n_dofs)
+    dst2 = torch.tensor(w, device=device)
+    op.vmult_add(dst2, src)
+    assert rel(dst2.cpu().numpy(), w + expect) < 1e-12
+    # plain (no constraints), used for the Dirichlet lifting
+    dst3 = torch.full_like(src, np.nan)
+    op.vmult(dst3, src, mask="plain")
+    assert rel(dst3.cpu().numpy(), ref.vmult(v, plain=True)) < 1e-12
+
+
+@pytest.mark.parametrize("dim,p,n", [(3, 2, 4), (2, 2, 5), (3, 4, 2)])
+@pytest.mark.parametrize("caching", ["scalar", "tensor2", "tensor4"])
+def test_vmult_fp32_level_operator(dim, p, n, caching, device):
+    import torch
+    log(f"[vmult fp32] dim={dim} p={p} n={n} caching={caching}")
+    mesh, mu, lam, u, cons = problem(dim, p, n)
+    ref = orc_op(mesh, mu, lam, u, cons, caching)
+    op = gpu_op(mesh, mu, lam, u, cons, caching, precision="fp32")
+    v = np.random.default_rng(1).random(mesh.n_dofs)
+    dst = torch.zeros(mesh.n_dofs, dtype=torch.float32, device=device)
+    op.vmult(dst, torch.tensor(v, dtype=torch.float32, device=device))
+    assert rel(dst.cpu().double().numpy(), ref.vmult(v.astype(np.float32).astype(np.float64))) < 2e-5
+
+
+@pytest.mark.parametrize("dim,p,n", [(3, 2, 3), (2, 2, 4), (3, 1, 3), (3, 4, 2)])
+@pytest.mark.parametrize("caching", ["scalar", "tensor2", "tensor4"])
+def test_diagonal_matches_oracle(dim, p, n, caching, device):
+    log(f"[diagonal] dim={dim} p={p} n={n} caching={caching}")
+    mesh, mu, lam, u, cons = problem(dim, p, n)
+    ref = orc_op(mesh, mu, lam, u, cons, caching).diagonal()
+    for prec, tol in [("fp64", 1e-12), ("fp32", 2e-5)]:
+        op = gpu_op(mesh, mu, lam, u, cons, caching, precision=prec)
+        op.compute_diagonal()
+        d = op.diagonal().cpu().double().numpy()
+        assert rel(d, ref) < tol, prec
+        inv = op.inverse_diagonal().cpu().double().numpy()
+        assert rel(inv, 1.0 / ref) < tol, prec
+        assert np.all(d[cons == 1] == 1.0)
+
+
+@pytest.mark.parametrize("dim,p,n", [(3, 2, 4), (2, 1, 6), (2, 2, 4), (3, 4, 2)])
+def test_residual_matches_oracle(dim, p, n, device):
+    import torch
+    log(f"[residual] dim={dim} p={p} n={n}")
+    mesh, mu, lam, u, cons = problem(dim, p, n, amplitude=0.05)
+    ref = orc_op(mesh, mu, lam, u, cons).residual()
+    op = gpu_op(mesh, mu, lam, u, cons, "tensor4")
+    r = torch.full((mesh.n_dofs,), np.nan, dtype=torch.float64, device=device)
+    op.residual(r)
+    assert rel(r.cpu().numpy(), ref) < 1e-12
+
+
+def test_cook_membrane_operator(device):
+    """the reference's Cook geometry (non-affine Q1 cells), Q1 and Q2"""
+    import torch
+    for p in (1, 2):
+        log(f"[cook] p={p}")
+        mesh = lsmf.Mesh(2, p, [16, 16], lo=[0, 0], hi=[48, 44], geometry="cook", scale=1e-3)
+        cons = mesh.constrained_mask({0: [0, 1]})
+        mu = np.full(mesh.n_cells, 0.4225e6)
+        lam = 2 * mu * 0.3 / 0.4
+        u, _ = lsmf.smooth_displacement(mesh.coords * 20, 0.001, 3)
+        for caching in ("scalar", "tensor2", "tensor4"):
+            ref = orc_op(mesh, mu, lam, u, cons, caching)
+            op = gpu_op(mesh, mu, lam, u, cons, caching)
+            v = np.random.default_rng(5).random(mesh.n_dofs)
+            dst = torch.zeros(mesh.n_dofs, dtype=torch.float64, device=device)
+            op.vmult(dst, torch.tensor(v, device=device))
+            assert rel(dst.cpu().numpy(), ref.vmult(v)) < 1e-12, caching
+
+
+def test_config1_full_size_tensor4(device):
+    """32^3 Q2 (823,875 DoF), 20 inclusions, smooth linearisation state"""
+    import torch
+    log("[config1] 32^3 Q2 tensor4 fp64 vs oracle")
+    mesh, mu, lam = lsmf.build_config_mesh(1)
+    u, _ = lsmf.smooth_displacement(mesh.coords, 0.02, lsmf.workload.SEED + 1)
+    cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
+    assert mesh.n_dofs == 823875
+    ref = orc_op(mesh, mu, lam, u, cons, "tensor4")
+    v = np.random.default_rng(8).random(mesh.n_dofs)
+    v[cons == 1] = 0.0
+    expect = ref.vmult(v)
+    del ref
+    for caching in ("tensor4", "tensor2", "scalar"):
+        op = gpu_op(mesh, mu, lam, u, cons, caching)
+        dst = torch.zeros(mesh.n_dofs, dtype=torch.float64, device=device)
+        op.vmult(dst, torch.tensor(v, device=device))
+        assert rel(dst.cpu().numpy(), expect) < 1e-12, caching
+        del op
+
+
+def test_nonpositive_det_F_is_reported(device):
+    import torch
+    mesh, mu, lam, u, cons = problem(3, 2, 2)
+    op = lsmf.NeoHookOperator(mesh, caching="tensor4", constrained=cons)
+    op.set_material(mu, lam)
+    op.set_linearization(torch.tensor(-2.0 * mesh.coords.reshape(-1), device=device))
+    with pytest.raises(lsmf.LsmfError) as e:
+        op.cache()
+    assert e.value.code == -4
+    with pytest.raises(lsmf.LsmfError) as e:
+        op.vmult(torch.zeros(mesh.n_dofs, dtype=torch.float64, device=device),
+                 torch.zeros(mesh.n_dofs, dtype=torch.float64, device=device))
+    assert e.value.code == -5
+
+
+def test_config2_full_size_properties(device):
+    """128^3 Q2 (50,923,779 DoF): symmetry, translations in the null space of
+    the unconstrained operator, tensor2 = tensor4 = scalar, add = set + set"""
+    import torch
+    log("[config2] 128^3 Q2 properties")
+    mesh, mu, lam = lsmf.build_config_mesh(2)
+    u, _ = lsmf.smooth_displacement(mesh.coords, 0.02, lsmf.workload.SEED + 2)
+    cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
+    assert mesh.n_dofs == 50923779
+    ud = torch.tensor(u, device=device)
+    del u
+    g = torch.Generator(device=device)
+    g.manual_seed(4)
+    v = torch.rand(mesh.n_dofs, dtype=torch.float64, device=device, generator=g)
+    w = torch.rand(mesh.n_dofs, dtype=torch.float64, device=device, generator=g)
+    res = {}
+    for caching in ("tensor4", "tensor2", "scalar"):
+        log(f"[config2] {caching}")
+        op = lsmf.NeoHookOperator(mesh, caching=caching, constrained=cons)
+        op.set_material(mu, lam)
+        op.set_linearization(ud)
+        op.cache()
+        Av = torch.empty_like(v)
+        op.vmult(Av, v)
+        if caching == "tensor4":
+            Aw = torch.empty_like(v)
+            op.vmult(Aw, w)
+            s1, s2 = float(w @ Av), float(v @ Aw)
+            assert abs(s1 - s2) < 1e-12 * abs(s1)
+            t = torch.tensor([0.3, -0.2, 0.5], dtype=torch.float64, device=device).repeat(mesh.n_nodes)
+            At = torch.empty_like(v)
+            op.vmult(At, t, mask="plain")
+            Avp = torch.empty_like(v)
+            op.vmult(Avp, v, mask="plain")
+            assert float(At.abs().max()) < 1e-11 * float(Avp.abs().max())
+            acc = Av.clone()
+            op.vmult_add(acc, v)
+            assert float((acc - 2 * Av).abs().max()) <= 1e-14 * float(Av.abs().max())
+        res[caching] = Av
+        del op
+    base = res["tensor4"]
+    for k in ("tensor2", "scalar"):
+        assert float((res[k] - base).abs().max()) < 1e-12 * float(base.abs().max()), k
