@@ -97,6 +97,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-n", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after timing, run 3 plain device copies of the source vector (known bytes) so "
+                         "HBM counters (FETCH_SIZE / WRITE_SIZE) can be scaled")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -149,6 +152,11 @@ def main():
     kern_ms, n_launch = op.profile_read()
     op.profile_enable(False)
     ms_per_step = total_ms / args.steps
+    if args.calibrate_copy:
+        progress(f"copy calibration: 3 x {x.numel() * x.element_size()} bytes read + written")
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
 
     nq = mesh.degree + 1
     n_qp = mesh.n_cells * nq ** mesh.dim
