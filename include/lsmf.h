@@ -211,6 +211,81 @@ int lsmf_mesh_boundary_faces(const lsmf_mesh *mesh, int side, int32_t *cells, in
 /* nodes on side s; returns the count (NULL output to query) */
 int lsmf_mesh_boundary_nodes(const lsmf_mesh *mesh, int side, int32_t *nodes);
 
+/* ------------------------------------------------------------------------
+ * Newton solve with GMG-preconditioned CG on a structured hierarchy.
+ * Replaces Solid::run / solve_nonlinear_timestep / setup_matrix_free /
+ * solve_linear_system (include/mf_elasticity.h:1187-1222, 1712-2241,
+ * 2245-2559, 3012-3214): FP64 outer CG on the fine operator, multigrid
+ * V-cycle with Chebyshev smoothing (Jacobi-preconditioned, eigenvalues by a
+ * CG/Lanczos estimate), Chebyshev used as the coarse solver, tensor-product
+ * transfer, level operators in FP32 or FP64 linearised at the displacement
+ * interpolated to each level.  Dirichlet values are increments applied at
+ * Newton iteration 0 of every load step (include/mf_elasticity.h:2963,
+ * 3204-3211); the lifting -K g enters the first residual.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  int dim, degree, n_q_1d;
+  int n_cells_coarse[3];         /* cells per direction of the coarsest level */
+  int n_levels;                  /* finest level has n_cells_coarse * 2^(n_levels-1) cells */
+  int geometry;                  /* LSMF_GEOMETRY_BOX / _COOK */
+  double lo[3], hi[3], scale;
+  int caching;                   /* fine (FP64) operator */
+  int mg_precision;              /* LSMF_FP32 or LSMF_FP64 level operators */
+  int mg_caching;
+  int smoother_degree;           /* Chebyshev degree, levels > 0 (5: include/mf_elasticity.h:2104) */
+  double smoothing_range;        /* 20 (:2102) */
+  int eig_cg_iterations;         /* 30 ("MF Chebyshev number CG iterations") */
+  double coarse_reduction;       /* 1e-3: coarse Chebyshev as a solver (:2087-2096) */
+  int coarse_eig_iterations;     /* 0: as many as coarse DoFs ("accurate eigenvalues") */
+  double tol_lin;                /* CG: |r| <= tol_lin |rhs| ("Residual", :3027) */
+  int max_lin_iterations;
+  int max_newton;                /* "Max iterations Newton-Raphson" */
+  double tol_f, tol_u;           /* "Tolerance force", "Tolerance displacement" */
+  double dt, t_end;              /* "Time step size", "End time" */
+  unsigned dirichlet_components[6]; /* per side s: bit c = component c prescribed */
+  double dirichlet_increment[6][3]; /* value per side and component (increment per load step) */
+  unsigned neumann_sides;            /* bit s: traction on side s */
+  double neumann_traction[6][3];     /* traction = t * value, reference configuration */
+  int verbose;
+} lsmf_solver_options;
+
+typedef struct {
+  int newton_iterations;  /* as printed by "Converged in N Newton iterations" */
+  int linear_iterations;  /* CG iterations summed over the step */
+  int max_linear_iterations;
+  double residual_0, residual_final;
+  double time_total_s, time_setup_s, time_linear_s, time_residual_s;
+} lsmf_step_stats;
+
+typedef struct lsmf_solver lsmf_solver;
+
+lsmf_status lsmf_solver_options_default(lsmf_solver_options *opt);
+/* mu, lambda: host arrays over the finest cells (mesh order); levels use the
+ * volume average of their children. */
+lsmf_status lsmf_solver_create(const lsmf_solver_options *opt, const double *mu, const double *lambda,
+                               lsmf_solver **out);
+lsmf_status lsmf_solver_destroy(lsmf_solver *s);
+/* load stepping t = dt, 2 dt, ... <= t_end; stats[k] per step */
+lsmf_status lsmf_solver_run(lsmf_solver *s, int max_steps, lsmf_step_stats *stats, int *n_steps);
+lsmf_status lsmf_solver_get_solution(const lsmf_solver *s, double *u_host);
+size_t lsmf_solver_n_dofs(const lsmf_solver *s, int level); /* level -1 = finest */
+/* linearise the fine operator and every level at u (host FP64, finest numbering) */
+lsmf_status lsmf_solver_set_state(lsmf_solver *s, const double *u_host);
+/* one V-cycle z = M r (device FP64 vectors of the finest level) */
+lsmf_status lsmf_solver_precondition(lsmf_solver *s, double *z_dev, const double *r_dev);
+/* FP64 PCG with the V-cycle: x = A^-1 b to |r| <= tol; x_dev starts at zero */
+lsmf_status lsmf_solver_cg(lsmf_solver *s, double *x_dev, const double *b_dev, double tol, int max_it, int *its,
+                           double *res);
+/* Chebyshev data of a level after set_state */
+lsmf_status lsmf_solver_level_info(const lsmf_solver *s, int level, double *lambda_max_est, double *lambda_min_est,
+                                   int *degree);
+/* FP64 transfer between level-1 and level on device vectors: mode 0
+ * prolongate (dst fine), 1 restrict = P^T (dst coarse), 2 interpolate the
+ * fine function at the coarse support points (dst coarse).  No constraints. */
+lsmf_status lsmf_solver_transfer(lsmf_solver *s, int level, int mode, double *dst, const double *src);
+/* the FP64 fine operator (borrowed) */
+lsmf_op *lsmf_solver_fine_operator(lsmf_solver *s);
+
 #ifdef __cplusplus
 }
 #endif

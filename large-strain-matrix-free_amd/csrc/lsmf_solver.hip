@@ -1,0 +1,979 @@
+// lsmf_solver.hip -- GMG-preconditioned CG and the Newton driver (host C++
+// over HIP kernels), C ABI part "lsmf_solver_*" of include/lsmf.h.
+//
+// Reference behaviour (include/mf_elasticity.h):
+//   * Newton loop, convergence test (>= 3 iterations; relative residual and
+//     update norms), Dirichlet increment at iteration 0 ......... :2245-2559
+//   * make_constraints: inhomogeneous only at it 0, levels homogeneous :2932-3007
+//   * residual incl. Neumann and the lifting of prescribed values .. :2708-2922
+//   * setup_matrix_free: level operators linearised at the displacement
+//     interpolated to each level, caches, diagonals, Chebyshev data,
+//     coarse Chebyshev as a solver, V-cycle ........................ :1712-2241
+//   * solve_linear_system: FP64 CG, tolerance relative to |rhs|,
+//     constraints.distribute of the update ......................... :3012-3214
+// deal.II algorithms restated: Multigrid::level_v_step (pre-smooth from zero,
+// residual, restrict, coarse, prolongate-add, post-smooth), PreconditionChebyshev
+// (vmult: degree-1 products from a zero start, step: degree products,
+// interval [1.2 lambda_max / range, 1.2 lambda_max], degree for a target
+// reduction when used as a solver), SolverCG's Lanczos eigenvalue estimate.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <type_traits>
+#include <vector>
+
+#include "lsmf.h"
+#include "lsmf_error.hpp"
+#include "lsmf_fe.hpp"
+#include "lsmf_solver.hpp"
+#include "lsmf_vec.hpp"
+
+namespace lsmf {
+
+namespace {
+inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+using Clock = std::chrono::steady_clock;
+inline double seconds_since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+}  // namespace
+
+// ---------------------------------------------------------------- Reducer
+Reducer::Reducer(hipStream_t s) : s_(s), part_(kRedBlocks * 2), res_(2) {}
+
+double Reducer::read(int nv, double *out) {
+  if (nv == 1)
+    hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kRedThreads), 0, s_, part_.p, res_.p);
+  else
+    hipLaunchKernelGGL(k_reduce_final<2>, dim3(1), dim3(kRedThreads), 0, s_, part_.p, res_.p);
+  hip_check(hipMemcpyAsync(out, res_.p, nv * sizeof(double), hipMemcpyDeviceToHost, s_), "reduce read");
+  hip_check(hipStreamSynchronize(s_), "reduce sync");
+  return out[0];
+}
+
+template <typename TA, typename TB> double Reducer::dot(const TA *a, const TB *b, size_t n) {
+  hipLaunchKernelGGL((k_dot<TA, TB>), dim3(kRedBlocks), dim3(kRedThreads), 0, s_, a, b, n, part_.p);
+  double r[2];
+  return read(1, r);
+}
+
+template <typename T>
+void Reducer::dot2(const T *a, const T *b, const T *c, const T *d, size_t n, double &r0, double &r1) {
+  hipLaunchKernelGGL(k_dot2<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, s_, a, b, c, d, n, part_.p);
+  double r[2];
+  read(2, r);
+  r0 = r[0];
+  r1 = r[1];
+}
+
+template <typename T> double Reducer::cg_update(T *x, T *r, const T *p, const T *Ap, double alpha, size_t n) {
+  hipLaunchKernelGGL(k_cg_update<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, s_, x, r, p, Ap, alpha, n, part_.p);
+  double rr[2];
+  return read(1, rr);
+}
+
+// ---------------------------------------------------------------- transfer
+// banded 1D operator: out[o] = sum_k w[o*W + k] in[start[o] + k]
+struct Band {
+  std::vector<int32_t> start;
+  std::vector<double> w;
+  int W = 0, n_out = 0, n_in = 0;
+};
+
+static double node_pos(const std::vector<double> &gl, int p, int n_cells, int i) {
+  const int k = std::min(i / p, n_cells - 1);
+  return k + gl[i - k * p];
+}
+
+// Q_p function on n_c coarse cells evaluated at the support points of 2 n_c
+// fine cells (MGTransferMatrixFree::prolongate, FE embedding)
+static Band prolongation_band(int p, int n_c) {
+  const std::vector<double> gl = lobatto_points(p);
+  Band b;
+  const int n_f = 2 * n_c;
+  b.n_out = n_f * p + 1;
+  b.n_in = n_c * p + 1;
+  b.W = p + 1;
+  b.start.resize(b.n_out);
+  b.w.resize((size_t)b.n_out * b.W);
+  for (int i = 0; i < b.n_out; ++i) {
+    const double x = 0.5 * node_pos(gl, p, n_f, i);  // coarse cell units
+    const int kc = std::min((int)std::floor(x), n_c - 1);
+    const double xi = x - kc;
+    b.start[i] = kc * p;
+    for (int a = 0; a <= p; ++a) b.w[(size_t)i * b.W + a] = lagrange_value(gl, a, xi);
+  }
+  return b;
+}
+
+// fine function evaluated at the coarse support points (interpolate_to_mg)
+static Band interpolation_band(int p, int n_c) {
+  const std::vector<double> gl = lobatto_points(p);
+  Band b;
+  const int n_f = 2 * n_c;
+  b.n_out = n_c * p + 1;
+  b.n_in = n_f * p + 1;
+  b.W = p + 1;
+  b.start.resize(b.n_out);
+  b.w.resize((size_t)b.n_out * b.W);
+  for (int I = 0; I < b.n_out; ++I) {
+    const double x = 2.0 * node_pos(gl, p, n_c, I);  // fine cell units
+    const int kf = std::min((int)std::floor(x + 1e-14), n_f - 1);
+    const double xi = x - kf;
+    b.start[I] = kf * p;
+    for (int a = 0; a <= p; ++a) b.w[(size_t)I * b.W + a] = lagrange_value(gl, a, xi);
+  }
+  return b;
+}
+
+static Band transpose_band(const Band &P) {
+  Band t;
+  t.n_out = P.n_in;
+  t.n_in = P.n_out;
+  std::vector<int> lo(t.n_out, 1 << 30), hi(t.n_out, -1);
+  for (int i = 0; i < P.n_out; ++i)
+    for (int k = 0; k < P.W; ++k) {
+      const int I = P.start[i] + k;
+      lo[I] = std::min(lo[I], i);
+      hi[I] = std::max(hi[I], i);
+    }
+  for (int I = 0; I < t.n_out; ++I) t.W = std::max(t.W, hi[I] - lo[I] + 1);
+  t.start.assign(t.n_out, 0);
+  t.w.assign((size_t)t.n_out * t.W, 0.0);
+  for (int I = 0; I < t.n_out; ++I) {
+    t.start[I] = std::min(lo[I], t.n_in - t.W);
+    t.start[I] = std::max(t.start[I], 0);
+  }
+  for (int i = 0; i < P.n_out; ++i)
+    for (int k = 0; k < P.W; ++k) {
+      const int I = P.start[i] + k;
+      t.w[(size_t)I * t.W + (i - t.start[I])] += P.w[(size_t)i * P.W + k];
+    }
+  return t;
+}
+
+struct DevBand {
+  DeviceArray<int32_t> start;
+  DeviceArray<double> w;
+  int W = 0, n_out = 0, n_in = 0;
+  void set(const Band &b) {
+    W = b.W;
+    n_out = b.n_out;
+    n_in = b.n_in;
+    start.resize(b.start.size());
+    start.upload(b.start.data(), b.start.size());
+    w.resize(b.w.size());
+    w.upload(b.w.data(), b.w.size());
+  }
+};
+
+// apply one band per direction (tensor product), in has node dims nin[0..dim)
+template <typename T>
+static void tensor_apply(const DevBand *const *bands, int dim, const int *nin, const T *in, T *out, T *tmpA, T *tmpB,
+                         bool add, hipStream_t s) {
+  int dims[3] = {nin[0], nin[1], dim == 3 ? nin[2] : 1};
+  const T *cur = in;
+  for (int d = 0; d < dim; ++d) {
+    T *dst = (d == dim - 1) ? out : ((d % 2 == 0) ? tmpA : tmpB);
+    const DevBand &b = *bands[d];
+    int od[3] = {dims[0], dims[1], dims[2]};
+    od[d] = b.n_out;
+    const size_t total = (size_t)od[0] * od[1] * od[2] * dim;
+    hipLaunchKernelGGL(k_apply_1d<T>, dim3(nblk(total)), dim3(256), 0, s, cur, dst, b.start.p, b.w.p, b.W, d, dims[0],
+                       dims[1], dims[2], b.n_out, dim, (add && d == dim - 1) ? 1 : 0);
+    dims[d] = b.n_out;
+    cur = dst;
+  }
+}
+
+// ---------------------------------------------------------------- levels
+static double sturm_count_below(const std::vector<double> &a, const std::vector<double> &b, double x) {
+  // number of eigenvalues of the symmetric tridiagonal (a diag, b offdiag) < x
+  int cnt = 0;
+  double q = a[0] - x;
+  if (q < 0) ++cnt;
+  for (size_t i = 1; i < a.size(); ++i) {
+    const double qq = (q != 0.0) ? q : 1e-300;
+    q = a[i] - x - b[i - 1] * b[i - 1] / qq;
+    if (q < 0) ++cnt;
+  }
+  return cnt;
+}
+
+static void tridiag_extremes(const std::vector<double> &a, const std::vector<double> &b, double &lmin, double &lmax) {
+  const size_t n = a.size();
+  double lo = 1e300, hi = -1e300;
+  for (size_t i = 0; i < n; ++i) {
+    const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  auto kth = [&](int k) {  // k-th smallest eigenvalue (0-based)
+    double l = lo, h = hi;
+    for (int it = 0; it < 200; ++it) {
+      const double m = 0.5 * (l + h);
+      if (sturm_count_below(a, b, m) > k) h = m;
+      else l = m;
+      if (h - l <= 1e-14 * std::max(std::fabs(l), std::fabs(h))) break;
+    }
+    return 0.5 * (l + h);
+  };
+  lmin = kth(0);
+  lmax = kth((int)n - 1);
+}
+
+struct GMGBase {
+  virtual ~GMGBase() {}
+  virtual void setup(const double *u_fine, double *tmp64a, double *tmp64b) = 0;
+  virtual void precondition(double *z, const double *r) = 0;
+  virtual void info(int level, double &lmax, double &lmin, int &degree) const = 0;
+  virtual int n_levels() const = 0;
+  virtual size_t level_dofs(int l) const = 0;
+  virtual void transfer64(int level, int mode, double *dst, const double *src) = 0;
+};
+
+struct LevelGeom {
+  lsmf_mesh *mesh = nullptr;
+  int nn[3] = {1, 1, 1};
+  size_t n_dofs = 0;
+  std::vector<uint8_t> cons;
+  std::vector<int32_t> cons_list_h;
+  DeviceArray<int32_t> cons_list;
+  std::vector<double> mu, lam;
+  ~LevelGeom() {
+    if (mesh) lsmf_mesh_destroy(mesh);
+  }
+};
+
+template <typename T> struct Level {
+  LevelGeom *g = nullptr;
+  std::unique_ptr<NeoHookOperatorHIP<T>> own;
+  NeoHookOperatorHIP<T> *op = nullptr;
+  DeviceArray<T> sol, rhs, t, d, z;
+  DeviceArray<double> u64;
+  DeviceArray<T> tmpA, tmpB;  // transfer temporaries (from level-1 to this level)
+  DevBand P[3], PT[3], R[3];
+  double lmax_est = 0, lmin_est = 0, theta = 0, delta = 0;
+  int degree = 0;
+};
+
+template <typename T> class GMG : public GMGBase {
+ public:
+  GMG(const lsmf_solver_options &o, std::vector<std::unique_ptr<LevelGeom>> &geoms, NeoHookOperatorHIP<double> *fine,
+      hipStream_t s)
+      : opt_(o), s_(s), red_(s) {
+    const int L = (int)geoms.size();
+    lv_.resize(L);
+    for (int l = 0; l < L; ++l) {
+      Level<T> &v = lv_[l];
+      v.g = geoms[l].get();
+      const size_t n = v.g->n_dofs;
+      if constexpr (std::is_same<T, double>::value) {
+        if (l == L - 1 && fine && o.mg_caching == o.caching) v.op = fine;
+      }
+      if (!v.op) {
+        v.own.reset(new NeoHookOperatorHIP<T>());
+        lsmf_mesh_desc md{o.dim, o.degree, v.g->mesh->n_cells, v.g->mesh->n_nodes, v.g->mesh->cell_nodes,
+                          v.g->mesh->coords, v.g->cons.data(), v.g->mesh->cell_colour};
+        v.own->initialize(md, o.n_q_1d, o.mg_caching, s);
+        v.own->set_material(v.g->mu.data(), v.g->lam.data());
+        v.op = v.own.get();
+      }
+      v.sol.resize(n);
+      v.rhs.resize(n);
+      v.t.resize(n);
+      v.d.resize(n);
+      v.z.resize(n);
+      if (l < L - 1) v.u64.resize(n);
+      if (l > 0) {
+        const LevelGeom &c = *geoms[l - 1];
+        size_t tmax = 0;
+        for (int d = 0; d < o.dim; ++d) {
+          const int nc = c.mesh->n_cells_dir[d];
+          Band P = prolongation_band(o.degree, nc);
+          v.P[d].set(P);
+          v.PT[d].set(transpose_band(P));
+          v.R[d].set(interpolation_band(o.degree, nc));
+        }
+        // largest intermediate of a prolongation / restriction sequence
+        int dims[3] = {c.nn[0], c.nn[1], o.dim == 3 ? c.nn[2] : 1};
+        for (int d = 0; d < o.dim; ++d) {
+          dims[d] = v.g->nn[d];
+          tmax = std::max(tmax, (size_t)dims[0] * dims[1] * dims[2] * o.dim);
+        }
+        int dims2[3] = {v.g->nn[0], v.g->nn[1], o.dim == 3 ? v.g->nn[2] : 1};
+        for (int d = 0; d < o.dim; ++d) {
+          dims2[d] = c.nn[d];
+          tmax = std::max(tmax, (size_t)dims2[0] * dims2[1] * dims2[2] * o.dim);
+        }
+        v.tmpA.resize(tmax);
+        v.tmpB.resize(tmax);
+        tmax64_ = std::max(tmax64_, tmax);
+      }
+    }
+  }
+
+  int n_levels() const override { return (int)lv_.size(); }
+  size_t level_dofs(int l) const override { return lv_[l].g->n_dofs; }
+  void info(int l, double &lmax, double &lmin, int &degree) const override {
+    lmax = lv_[l].lmax_est;
+    lmin = lv_[l].lmin_est;
+    degree = lv_[l].degree;
+  }
+  size_t tmax64() const { return tmax64_; }
+
+  void prolongate_add(int l, T *fine, const T *coarse) {
+    Level<T> &v = lv_[l];
+    const DevBand *b[3] = {&v.P[0], &v.P[1], &v.P[2]};
+    tensor_apply<T>(b, opt_.dim, lv_[l - 1].g->nn, coarse, fine, v.tmpA.p, v.tmpB.p, true, s_);
+  }
+  void restrict_(int l, T *coarse, const T *fine) {
+    Level<T> &v = lv_[l];
+    const DevBand *b[3] = {&v.PT[0], &v.PT[1], &v.PT[2]};
+    tensor_apply<T>(b, opt_.dim, v.g->nn, fine, coarse, v.tmpA.p, v.tmpB.p, false, s_);
+    zero_constrained(l - 1, coarse);
+  }
+  void zero_constrained(int l, T *x) {
+    const LevelGeom &g = *lv_[l].g;
+    if (!g.cons_list_h.empty())
+      hipLaunchKernelGGL(k_set_list<T>, dim3(nblk(g.cons_list_h.size())), dim3(256), 0, s_, x, g.cons_list.p,
+                         (const double *)nullptr, 0.0, (int)g.cons_list_h.size());
+  }
+
+  void transfer64(int l, int mode, double *dst, const double *src) override {
+    Level<T> &v = lv_[l];
+    DeviceArray<double> a(tmax64_), b(tmax64_);
+    if (mode == 0) {
+      const DevBand *bb[3] = {&v.P[0], &v.P[1], &v.P[2]};
+      tensor_apply<double>(bb, opt_.dim, lv_[l - 1].g->nn, src, dst, a.p, b.p, false, s_);
+    } else {
+      const DevBand *bb[3] = {mode == 1 ? &v.PT[0] : &v.R[0], mode == 1 ? &v.PT[1] : &v.R[1],
+                              mode == 1 ? &v.PT[2] : &v.R[2]};
+      tensor_apply<double>(bb, opt_.dim, v.g->nn, src, dst, a.p, b.p, false, s_);
+    }
+    hip_check(hipStreamSynchronize(s_), "transfer");
+  }
+
+  // linearise every level at the fine displacement: interpolate_to_mg
+  // (include/mf_elasticity.h:1926-1942), level caches and diagonals
+  // (:2055-2073), Chebyshev data (:2081-2113)
+  void setup(const double *u_fine, double *ta, double *tb) override {
+    const int L = (int)lv_.size();
+    const double *ucur = u_fine;
+    for (int l = L - 1; l >= 0; --l) {
+      Level<T> &v = lv_[l];
+      if (l < L - 1) {
+        Level<T> &f = lv_[l + 1];
+        const DevBand *bb[3] = {&f.R[0], &f.R[1], &f.R[2]};
+        tensor_apply<double>(bb, opt_.dim, f.g->nn, ucur, v.u64.p, ta, tb, false, s_);
+        ucur = v.u64.p;
+      }
+      if (v.own) {
+        v.op->set_linearization(ucur);
+        v.op->cache();
+      }
+      v.op->compute_diagonal();
+      if (l == 0) {
+        const int its = opt_.coarse_eig_iterations > 0 ? opt_.coarse_eig_iterations : (int)v.g->n_dofs;
+        estimate_eigenvalues(v, its);
+        const double lmax = 1.2 * v.lmax_est;
+        const double alpha = std::min(0.9 * lmax, v.lmin_est);
+        v.theta = 0.5 * (lmax + alpha);
+        v.delta = 0.5 * (lmax - alpha);
+        const double range = lmax / alpha;
+        const double sigma = (1.0 - std::sqrt(1.0 / range)) / (1.0 + std::sqrt(1.0 / range));
+        const double eps = opt_.coarse_reduction;
+        v.degree = 1 + (int)(std::log(1.0 / eps + std::sqrt(1.0 / eps / eps - 1.0)) / std::log(1.0 / sigma));
+        v.degree = std::min(std::max(v.degree, 1), 2000);
+      } else {
+        estimate_eigenvalues(v, opt_.eig_cg_iterations);
+        const double lmax = 1.2 * v.lmax_est;
+        const double alpha = lmax / opt_.smoothing_range;
+        v.theta = 0.5 * (lmax + alpha);
+        v.delta = 0.5 * (lmax - alpha);
+        v.degree = opt_.smoother_degree;
+      }
+    }
+  }
+
+  // z = M r, the V-cycle preconditioner between FP64 vectors of the finest
+  // level (copy_to_mg / copy_from_mg convert FP64 <-> level precision)
+  void precondition(double *z, const double *r) override {
+    const int L = (int)lv_.size() - 1;
+    Level<T> &f = lv_[L];
+    const size_t n = f.g->n_dofs;
+    hipLaunchKernelGGL((k_convert<double, T>), dim3(nblk(n)), dim3(256), 0, s_, r, f.rhs.p, n);
+    vcycle(L);
+    hipLaunchKernelGGL((k_convert<T, double>), dim3(nblk(n)), dim3(256), 0, s_, f.sol.p, z, n);
+  }
+
+ private:
+  // Chebyshev smoother / coarse solver on level v: x from zero (vmult) or
+  // from its current value (step)
+  void chebyshev(Level<T> &v, T *x, const T *b, bool x_zero) {
+    const size_t n = v.g->n_dofs;
+    const T *invd = v.op->get_matrix_diagonal_inverse();
+    if (x_zero) {
+      hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(256), 0, s_, x, v.d.p, b, (const T *)nullptr, invd,
+                         1.0 / v.theta, 1, n);
+    } else {
+      v.op->vmult(v.t.p, x);
+      hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(256), 0, s_, x, v.d.p, b, (const T *)v.t.p, invd,
+                         1.0 / v.theta, 0, n);
+    }
+    if (v.degree < 2 || std::fabs(v.delta) < 1e-40) return;
+    double rhok = v.delta / v.theta;
+    const double sigma = v.theta / v.delta;
+    for (int k = 0; k < v.degree - 1; ++k) {
+      v.op->vmult(v.t.p, x);
+      const double rhokp = 1.0 / (2.0 * sigma - rhok);
+      const double f1 = rhokp * rhok, f2 = 2.0 * rhokp / v.delta;
+      rhok = rhokp;
+      hipLaunchKernelGGL(k_cheb_update<T>, dim3(nblk(n)), dim3(256), 0, s_, x, v.d.p, b, (const T *)v.t.p, invd, f1, f2,
+                         n);
+    }
+  }
+
+  // Multigrid::level_v_step
+  void vcycle(int l) {
+    Level<T> &v = lv_[l];
+    const size_t n = v.g->n_dofs;
+    if (l == 0) {
+      chebyshev(v, v.sol.p, v.rhs.p, true);
+      return;
+    }
+    chebyshev(v, v.sol.p, v.rhs.p, true);  // pre-smoothing from zero
+    v.op->vmult(v.t.p, v.sol.p);
+    hipLaunchKernelGGL(k_residual<T>, dim3(nblk(n)), dim3(256), 0, s_, v.t.p, (const T *)v.rhs.p, n);
+    Level<T> &c = lv_[l - 1];
+    restrict_(l, c.rhs.p, v.t.p);
+    vcycle(l - 1);
+    prolongate_add(l, v.sol.p, c.sol.p);
+    zero_constrained(l, v.sol.p);
+    chebyshev(v, v.sol.p, v.rhs.p, false);  // post-smoothing
+  }
+
+  // CG on D^-1 A with the Lanczos tridiagonal of its coefficients
+  // (SolverCG's condition-number / eigenvalue estimate)
+  void estimate_eigenvalues(Level<T> &v, int its) {
+    const size_t n = v.g->n_dofs;
+    T *x = v.sol.p, *r = v.rhs.p, *p = v.d.p, *Ap = v.t.p, *z = v.z.p;
+    const T *invd = v.op->get_matrix_diagonal_inverse();
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, s_, x, T(0), n);
+    hipLaunchKernelGGL(k_pseudo_random<T>, dim3(nblk(n)), dim3(256), 0, s_, r, n, (uint64_t)20261202ull);
+    zero_constrained((int)(&v - lv_.data()), r);
+    hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(n)), dim3(256), 0, s_, z, (const T *)r, invd, n);
+    hip_check(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, s_), "eig copy");
+    double rz = red_.dot(r, z, n);
+    const double r0 = std::sqrt(red_.dot(r, r, n));
+    std::vector<double> alpha, beta;
+    for (int k = 0; k < its; ++k) {
+      v.op->vmult(Ap, p);
+      const double pAp = red_.dot(p, Ap, n);
+      if (!(pAp > 0)) break;
+      const double a = rz / pAp;
+      alpha.push_back(a);
+      const double rr = red_.cg_update(x, r, p, Ap, a, n);
+      if (std::sqrt(rr) <= 1e-10 * r0) break;
+      hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(n)), dim3(256), 0, s_, z, (const T *)r, invd, n);
+      const double rz_new = red_.dot(r, z, n);
+      const double b = rz_new / rz;
+      beta.push_back(b);
+      rz = rz_new;
+      hipLaunchKernelGGL(k_xpby<T>, dim3(nblk(n)), dim3(256), 0, s_, p, (const T *)z, b, n);
+    }
+    const size_t m = alpha.size();
+    if (m == 0) throw HipError(LSMF_ERR_HIP, "eigenvalue estimate: no CG step");
+    std::vector<double> diag(m), off(m > 1 ? m - 1 : 0);
+    diag[0] = 1.0 / alpha[0];
+    for (size_t k = 1; k < m; ++k) {
+      diag[k] = 1.0 / alpha[k] + beta[k - 1] / alpha[k - 1];
+      off[k - 1] = std::sqrt(beta[k - 1]) / alpha[k - 1];
+    }
+    if (m == 1) {
+      v.lmax_est = v.lmin_est = diag[0];
+    } else {
+      tridiag_extremes(diag, off, v.lmin_est, v.lmax_est);
+    }
+  }
+
+  lsmf_solver_options opt_;
+  hipStream_t s_;
+  Reducer red_;
+  std::vector<Level<T>> lv_;
+  size_t tmax64_ = 0;
+};
+
+}  // namespace lsmf
+
+using namespace lsmf;
+
+struct lsmf_solver {
+  lsmf_solver_options opt{};
+  hipStream_t s = nullptr;
+  std::vector<std::unique_ptr<LevelGeom>> geoms;  // coarsest .. finest
+  NeoHookOperatorHIP<double> fine;
+  std::unique_ptr<GMGBase> mg;
+  std::unique_ptr<Reducer> red;
+  size_t n = 0;
+  DeviceArray<double> u_n, delta, u_tot, rhs, du, g, Ag, neumann, r, z, p, Ap, ta, tb;
+  std::vector<int32_t> cons_list;
+  std::vector<double> cons_inc;  // increment per constrained DoF
+  DeviceArray<int32_t> d_cons_list;
+  DeviceArray<double> d_cons_inc;
+  bool has_increment = false, has_neumann = false;
+  ~lsmf_solver() {
+    mg.reset();
+    fine.clear();
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+namespace {
+
+void build_level_geom(const lsmf_solver_options &o, int level, LevelGeom &g) {
+  int nc[3] = {1, 1, 1};
+  for (int d = 0; d < o.dim; ++d) nc[d] = o.n_cells_coarse[d] << level;
+  abi_check(lsmf_mesh_create(o.dim, o.degree, nc, o.lo, o.hi, o.geometry, o.scale, &g.mesh), "mesh");
+  for (int d = 0; d < o.dim; ++d) g.nn[d] = nc[d] * o.degree + 1;
+  g.n_dofs = (size_t)g.mesh->n_nodes * o.dim;
+  g.cons.assign(g.n_dofs, 0);
+  for (int s = 0; s < 2 * o.dim; ++s) {
+    if (!o.dirichlet_components[s]) continue;
+    const int cnt = lsmf_mesh_boundary_nodes(g.mesh, s, nullptr);
+    std::vector<int32_t> nodes(cnt);
+    lsmf_mesh_boundary_nodes(g.mesh, s, nodes.data());
+    for (int nd : nodes)
+      for (int c = 0; c < o.dim; ++c)
+        if ((o.dirichlet_components[s] >> c) & 1u) g.cons[(size_t)nd * o.dim + c] = 1;
+  }
+  for (size_t i = 0; i < g.n_dofs; ++i)
+    if (g.cons[i]) g.cons_list_h.push_back((int32_t)i);
+  g.cons_list.resize(g.cons_list_h.size());
+  if (!g.cons_list_h.empty()) g.cons_list.upload(g.cons_list_h.data(), g.cons_list_h.size());
+}
+
+// cell (i,j,k) of level l-1 averages its 2^dim children of level l
+void coarsen_material(int dim, const int *nc_fine, const std::vector<double> &f, std::vector<double> &c) {
+  const int n0 = nc_fine[0], n1 = nc_fine[1], n2 = dim == 3 ? nc_fine[2] : 1;
+  const int m0 = n0 / 2, m1 = n1 / 2, m2 = dim == 3 ? n2 / 2 : 1;
+  c.assign((size_t)m0 * m1 * m2, 0.0);
+  const double w = 1.0 / (1 << dim);
+  for (int k = 0; k < n2; ++k)
+    for (int j = 0; j < n1; ++j)
+      for (int i = 0; i < n0; ++i)
+        c[(size_t)(i / 2) + (size_t)m0 * ((j / 2) + (size_t)m1 * (dim == 3 ? k / 2 : 0))] +=
+            w * f[(size_t)i + (size_t)n0 * (j + (size_t)n1 * k)];
+}
+
+// Neumann load for t = 1 on the finest mesh (reference configuration)
+std::vector<double> neumann_vector(const lsmf_solver_options &o, const LevelGeom &g) {
+  std::vector<double> f(g.n_dofs, 0.0);
+  if (!o.neumann_sides) return f;
+  Tables1D t;
+  t.init(o.degree, o.n_q_1d);
+  const int dim = o.dim, p = o.degree, N = p + 1;
+  const int npc = dim == 3 ? N * N * N : N * N;
+  for (int s = 0; s < 2 * dim; ++s) {
+    if (!((o.neumann_sides >> s) & 1u)) continue;
+    const int cnt = lsmf_mesh_boundary_faces(g.mesh, s, nullptr, nullptr);
+    std::vector<int32_t> cells(cnt), faces(cnt);
+    lsmf_mesh_boundary_faces(g.mesh, s, cells.data(), faces.data());
+    const int fd = s / 2, side = s % 2;
+    for (int k = 0; k < cnt; ++k) {
+      const int32_t *cn = g.mesh->cell_nodes + (size_t)cells[k] * npc;
+      double X[8][3] = {};
+      for (int v = 0; v < (1 << dim); ++v) {
+        int a = 0, st = 1;
+        for (int e = 0; e < dim; ++e) {
+          a += ((v >> e) & 1) * p * st;
+          st *= N;
+        }
+        for (int i = 0; i < dim; ++i) X[v][i] = g.mesh->coords[(size_t)cn[a] * dim + i];
+      }
+      const int nfq = dim == 3 ? o.n_q_1d * o.n_q_1d : o.n_q_1d;
+      for (int fq = 0; fq < nfq; ++fq) {
+        double xi[3] = {0, 0, 0}, w = 1.0;
+        int r = fq;
+        for (int d = 0; d < dim; ++d) {
+          if (d == fd) {
+            xi[d] = side;
+            continue;
+          }
+          xi[d] = t.qpts[r % o.n_q_1d];
+          w *= t.qw[r % o.n_q_1d];
+          r /= o.n_q_1d;
+        }
+        double J[3][3] = {};
+        for (int v = 0; v < (1 << dim); ++v)
+          for (int d = 0; d < dim; ++d) {
+            double gv = 1.0;
+            for (int e = 0; e < dim; ++e) {
+              const int ve = (v >> e) & 1;
+              gv *= (e == d) ? (ve ? 1.0 : -1.0) : (ve ? xi[e] : 1.0 - xi[e]);
+            }
+            for (int i = 0; i < dim; ++i) J[i][d] += X[v][i] * gv;
+          }
+        // face measure: |cofactor column of J for direction fd|
+        double meas;
+        if (dim == 2) {
+          const int td = 1 - fd;
+          meas = std::sqrt(J[0][td] * J[0][td] + J[1][td] * J[1][td]);
+        } else {
+          const int a1 = (fd + 1) % 3, a2 = (fd + 2) % 3;
+          const double cx = J[1][a1] * J[2][a2] - J[2][a1] * J[1][a2];
+          const double cy = J[2][a1] * J[0][a2] - J[0][a1] * J[2][a2];
+          const double cz = J[0][a1] * J[1][a2] - J[1][a1] * J[0][a2];
+          meas = std::sqrt(cx * cx + cy * cy + cz * cz);
+        }
+        const double jxw = meas * w;
+        for (int a = 0; a < npc; ++a) {
+          int ai = a;
+          double phi = 1.0;
+          for (int d = 0; d < dim; ++d) {
+            phi *= lagrange_value(t.nodes, ai % N, xi[d]);
+            ai /= N;
+          }
+          for (int c = 0; c < dim; ++c) f[(size_t)cn[a] * dim + c] += phi * o.neumann_traction[s][c] * jxw;
+        }
+      }
+    }
+  }
+  return f;
+}
+
+int catch_all(const std::function<void()> &fn) {
+  try {
+    fn();
+    return LSMF_OK;
+  } catch (const HipError &e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(LSMF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(LSMF_ERR_INVALID_ARGUMENT, e.what());
+  }
+}
+
+// FP64 PCG with the V-cycle (SolverCG::solve with PreconditionMG)
+int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, double *res_out) {
+  const size_t n = S->n;
+  hipStream_t s = S->s;
+  Reducer &red = *S->red;
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, s, x, 0.0, n);
+  hip_check(hipMemcpyAsync(S->r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, s), "pcg copy");
+  double rn = std::sqrt(red.dot(S->r.p, S->r.p, n));
+  if (rn <= tol) {
+    *res_out = rn;
+    return 0;
+  }
+  S->mg->precondition(S->z.p, S->r.p);
+  hip_check(hipMemcpyAsync(S->p.p, S->z.p, n * sizeof(double), hipMemcpyDeviceToDevice, s), "pcg copy");
+  double rz = red.dot(S->r.p, S->z.p, n);
+  int it = 0;
+  while (it < max_it) {
+    ++it;
+    S->fine.vmult(S->Ap.p, S->p.p);
+    const double pAp = red.dot(S->p.p, S->Ap.p, n);
+    const double alpha = rz / pAp;
+    const double rr = red.cg_update(x, S->r.p, (const double *)S->p.p, (const double *)S->Ap.p, alpha, n);
+    rn = std::sqrt(rr);
+    if (rn <= tol) break;
+    S->mg->precondition(S->z.p, S->r.p);
+    const double rz_new = red.dot(S->r.p, S->z.p, n);
+    const double beta = rz_new / rz;
+    rz = rz_new;
+    hipLaunchKernelGGL(k_xpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->p.p, (const double *)S->z.p, beta, n);
+  }
+  *res_out = rn;
+  return rn <= tol ? it : -it;
+}
+
+void set_state(lsmf_solver *S, const double *u_dev) {
+  S->fine.set_linearization(u_dev);
+  S->fine.cache();
+  S->mg->setup(u_dev, S->ta.p, S->tb.p);
+}
+
+}  // namespace
+
+extern "C" {
+
+lsmf_status lsmf_solver_options_default(lsmf_solver_options *o) {
+  if (!o) return fail(LSMF_ERR_INVALID_ARGUMENT, "options: null");
+  std::memset(o, 0, sizeof(*o));
+  o->dim = 3;
+  o->degree = 2;
+  o->n_q_1d = 3;
+  o->n_cells_coarse[0] = o->n_cells_coarse[1] = o->n_cells_coarse[2] = 2;
+  o->n_levels = 3;
+  o->geometry = LSMF_GEOMETRY_BOX;
+  o->hi[0] = o->hi[1] = o->hi[2] = 1.0;
+  o->scale = 1.0;
+  o->caching = LSMF_CACHING_TENSOR4;
+  o->mg_precision = LSMF_FP32;
+  o->mg_caching = LSMF_CACHING_TENSOR4;
+  o->smoother_degree = 5;
+  o->smoothing_range = 20.0;
+  o->eig_cg_iterations = 30;
+  o->coarse_reduction = 1e-3;
+  o->coarse_eig_iterations = 0;
+  o->tol_lin = 1e-6;
+  o->max_lin_iterations = 1000;
+  o->max_newton = 10;
+  o->tol_f = 1e-9;
+  o->tol_u = 1e-6;
+  o->dt = 0.1;
+  o->t_end = 1.0;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_solver_create(const lsmf_solver_options *o, const double *mu, const double *lambda,
+                               lsmf_solver **out) {
+  if (!o || !mu || !lambda || !out) return fail(LSMF_ERR_INVALID_ARGUMENT, "solver_create: null argument");
+  if ((o->dim != 2 && o->dim != 3) || o->n_levels < 1 || o->n_levels > 16)
+    return fail(LSMF_ERR_INVALID_ARGUMENT, "solver_create: bad dim or n_levels");
+  if (o->mg_precision != LSMF_FP32 && o->mg_precision != LSMF_FP64)
+    return fail(LSMF_ERR_INVALID_ARGUMENT, "solver_create: bad mg_precision");
+  *out = nullptr;
+  lsmf_solver *S = new (std::nothrow) lsmf_solver;
+  if (!S) return fail(LSMF_ERR_OUT_OF_MEMORY, "solver_create: host allocation");
+  const int rc = catch_all([&] {
+    S->opt = *o;
+    hip_check(hipStreamCreate(&S->s), "hipStreamCreate");
+    const int L = o->n_levels;
+    S->geoms.resize(L);
+    for (int l = 0; l < L; ++l) {
+      S->geoms[l].reset(new LevelGeom);
+      build_level_geom(*o, l, *S->geoms[l]);
+    }
+    LevelGeom &fg = *S->geoms[L - 1];
+    fg.mu.assign(mu, mu + fg.mesh->n_cells);
+    fg.lam.assign(lambda, lambda + fg.mesh->n_cells);
+    for (int l = L - 1; l > 0; --l) {
+      coarsen_material(o->dim, S->geoms[l]->mesh->n_cells_dir, S->geoms[l]->mu, S->geoms[l - 1]->mu);
+      coarsen_material(o->dim, S->geoms[l]->mesh->n_cells_dir, S->geoms[l]->lam, S->geoms[l - 1]->lam);
+    }
+    S->n = fg.n_dofs;
+    lsmf_mesh_desc md{o->dim, o->degree, fg.mesh->n_cells, fg.mesh->n_nodes, fg.mesh->cell_nodes, fg.mesh->coords,
+                      fg.cons.data(), fg.mesh->cell_colour};
+    S->fine.initialize(md, o->n_q_1d, o->caching, S->s);
+    S->fine.set_material(fg.mu.data(), fg.lam.data());
+    if (o->mg_precision == LSMF_FP32)
+      S->mg.reset(new GMG<float>(*o, S->geoms, &S->fine, S->s));
+    else
+      S->mg.reset(new GMG<double>(*o, S->geoms, &S->fine, S->s));
+    S->red.reset(new Reducer(S->s));
+    const size_t n = S->n;
+    for (DeviceArray<double> *v : {&S->u_n, &S->delta, &S->u_tot, &S->rhs, &S->du, &S->g, &S->Ag, &S->neumann, &S->r,
+                                   &S->z, &S->p, &S->Ap})
+      v->resize(n);
+    size_t tmax = 0;
+    for (int l = 1; l < L; ++l) tmax = std::max(tmax, S->geoms[l]->n_dofs * 2);
+    S->ta.resize(std::max<size_t>(tmax, 1));
+    S->tb.resize(std::max<size_t>(tmax, 1));
+    hip_check(hipMemset(S->u_n.p, 0, n * sizeof(double)), "memset");
+    // constrained DoFs of the finest level and their increments
+    for (size_t i = 0; i < n; ++i)
+      if (fg.cons[i]) S->cons_list.push_back((int32_t)i);
+    S->cons_inc.assign(S->cons_list.size(), 0.0);
+    for (int s = 0; s < 2 * o->dim; ++s) {
+      if (!o->dirichlet_components[s]) continue;
+      const int cnt = lsmf_mesh_boundary_nodes(fg.mesh, s, nullptr);
+      std::vector<int32_t> nodes(cnt);
+      lsmf_mesh_boundary_nodes(fg.mesh, s, nodes.data());
+      for (int nd : nodes)
+        for (int c = 0; c < o->dim; ++c)
+          if ((o->dirichlet_components[s] >> c) & 1u) {
+            const size_t dof = (size_t)nd * o->dim + c;
+            const auto it = std::lower_bound(S->cons_list.begin(), S->cons_list.end(), (int32_t)dof);
+            S->cons_inc[it - S->cons_list.begin()] = o->dirichlet_increment[s][c];
+            if (o->dirichlet_increment[s][c] != 0.0) S->has_increment = true;
+          }
+    }
+    S->d_cons_list.resize(S->cons_list.size());
+    S->d_cons_inc.resize(S->cons_inc.size());
+    if (!S->cons_list.empty()) {
+      S->d_cons_list.upload(S->cons_list.data(), S->cons_list.size());
+      S->d_cons_inc.upload(S->cons_inc.data(), S->cons_inc.size());
+    }
+    std::vector<double> f = neumann_vector(*o, fg);
+    S->neumann.upload(f.data(), n);
+    S->has_neumann = o->neumann_sides != 0;
+  });
+  if (rc) {
+    delete S;
+    return rc;
+  }
+  *out = S;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_solver_destroy(lsmf_solver *s) {
+  delete s;
+  return LSMF_OK;
+}
+
+size_t lsmf_solver_n_dofs(const lsmf_solver *s, int level) {
+  if (!s) return 0;
+  if (level < 0) return s->n;
+  if (level >= (int)s->geoms.size()) return 0;
+  return s->geoms[level]->n_dofs;
+}
+
+lsmf_op *lsmf_solver_fine_operator(lsmf_solver *s) { return s ? s->fine.handle() : nullptr; }
+
+lsmf_status lsmf_solver_set_state(lsmf_solver *S, const double *u_host) {
+  if (!S || !u_host) return fail(LSMF_ERR_INVALID_ARGUMENT, "set_state: null argument");
+  return catch_all([&] {
+    S->u_tot.upload(u_host, S->n);
+    set_state(S, S->u_tot.p);
+    hip_check(hipStreamSynchronize(S->s), "set_state");
+  });
+}
+
+lsmf_status lsmf_solver_precondition(lsmf_solver *S, double *z, const double *r) {
+  if (!S || !z || !r) return fail(LSMF_ERR_INVALID_ARGUMENT, "precondition: null argument");
+  return catch_all([&] {
+    S->mg->precondition(z, r);
+    hip_check(hipStreamSynchronize(S->s), "precondition");
+  });
+}
+
+lsmf_status lsmf_solver_cg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, int *its,
+                           double *res) {
+  if (!S || !x || !b || !its || !res) return fail(LSMF_ERR_INVALID_ARGUMENT, "cg: null argument");
+  return catch_all([&] {
+    const int k = pcg(S, x, b, tol, max_it, res);
+    *its = k < 0 ? -k : k;
+    if (k < 0) throw HipError(LSMF_ERR_NOT_CONVERGED, "cg: no convergence");
+  });
+}
+
+lsmf_status lsmf_solver_level_info(const lsmf_solver *S, int level, double *lmax, double *lmin, int *degree) {
+  if (!S || !lmax || !lmin || !degree || level < 0 || level >= S->mg->n_levels())
+    return fail(LSMF_ERR_INVALID_ARGUMENT, "level_info: bad argument");
+  S->mg->info(level, *lmax, *lmin, *degree);
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_solver_transfer(lsmf_solver *S, int level, int mode, double *dst, const double *src) {
+  if (!S || !dst || !src || level < 1 || level >= (int)S->geoms.size() || mode < 0 || mode > 2)
+    return fail(LSMF_ERR_INVALID_ARGUMENT, "transfer: bad argument");
+  return catch_all([&] { S->mg->transfer64(level, mode, dst, src); });
+}
+
+lsmf_status lsmf_solver_get_solution(const lsmf_solver *S, double *u_host) {
+  if (!S || !u_host) return fail(LSMF_ERR_INVALID_ARGUMENT, "get_solution: null argument");
+  return catch_all([&] {
+    hip_check(hipStreamSynchronize(S->s), "sync");
+    hip_check(hipMemcpy(u_host, S->u_n.p, S->n * sizeof(double), hipMemcpyDeviceToHost), "get_solution");
+  });
+}
+
+// Solid::run with solve_nonlinear_timestep (include/mf_elasticity.h:1187-1222, 2289-2559)
+lsmf_status lsmf_solver_run(lsmf_solver *S, int max_steps, lsmf_step_stats *stats, int *n_steps) {
+  if (!S || !n_steps) return fail(LSMF_ERR_INVALID_ARGUMENT, "run: null argument");
+  *n_steps = 0;
+  return catch_all([&] {
+    const lsmf_solver_options &o = S->opt;
+    const size_t n = S->n;
+    hipStream_t s = S->s;
+    Reducer &red = *S->red;
+    const int ncons = (int)S->cons_list.size();
+    double t = 0.0;
+    t += o.dt;
+    int step = 0;
+    while (t <= o.t_end && step < max_steps) {
+      const auto t_step = Clock::now();
+      lsmf_step_stats st{};
+      hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, s, S->delta.p, 0.0, n);
+      double res0 = 0, upd0 = 0, upd_prev = 0;
+      bool converged = false;
+      int it = 0;
+      for (; it < o.max_newton; ++it) {
+        auto t0 = Clock::now();
+        // set_total_solution + cache at the new linearisation point
+        hipLaunchKernelGGL(k_add<double>, dim3(nblk(n)), dim3(256), 0, s, S->u_tot.p, (const double *)S->u_n.p,
+                           (const double *)S->delta.p, n);
+        S->fine.set_linearization(S->u_tot.p);
+        S->fine.cache();
+        st.time_setup_s += seconds_since(t0);
+        t0 = Clock::now();
+        // assemble_system: residual + Neumann + lifting (-K g) at it 0
+        abi_check(lsmf_residual(S->fine.handle(), S->rhs.p, s), "residual");
+        if (S->has_neumann)
+          hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->rhs.p, (const double *)S->neumann.p,
+                             t, 1.0, n);
+        const bool lift = (it == 0 && S->has_increment);
+        if (lift) {
+          hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, s, S->g.p, 0.0, n);
+          hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->g.p, S->d_cons_list.p,
+                             (const double *)S->d_cons_inc.p, 0.0, ncons);
+          S->fine.vmult(S->Ag.p, S->g.p, LSMF_MASK_PLAIN);
+          hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->rhs.p, (const double *)S->Ag.p, -1.0,
+                             1.0, n);
+        }
+        if (ncons)
+          hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->rhs.p, S->d_cons_list.p,
+                             (const double *)nullptr, 0.0, ncons);
+        const double res = std::sqrt(red.dot(S->rhs.p, S->rhs.p, n));
+        st.time_residual_s += seconds_since(t0);
+        if (it == 0) {
+          res0 = res;
+          st.residual_0 = res;
+        }
+        st.residual_final = res;
+        // check_convergence (include/mf_elasticity.h:2245-2284)
+        if (it > 2) {
+          const double rn = res0 > 0 ? res / res0 : res;
+          const double un = upd0 > 0 ? upd_prev / upd0 : upd_prev;
+          if (res <= 0.0 || upd_prev <= 0.0 || (rn <= o.tol_f && un <= o.tol_u)) converged = true;
+        }
+        if (converged) break;
+        t0 = Clock::now();
+        S->mg->setup(S->u_tot.p, S->ta.p, S->tb.p);
+        hip_check(hipStreamSynchronize(s), "setup sync");
+        st.time_setup_s += seconds_since(t0);
+        t0 = Clock::now();
+        double lin_res = 0;
+        const int lin = pcg(S, S->du.p, S->rhs.p, o.tol_lin * res, o.max_lin_iterations, &lin_res);
+        st.time_linear_s += seconds_since(t0);
+        if (lin < 0) throw HipError(LSMF_ERR_NOT_CONVERGED, "linear solver did not converge");
+        st.linear_iterations += lin;
+        st.max_linear_iterations = std::max(st.max_linear_iterations, lin);
+        if (ncons)
+          hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->du.p, S->d_cons_list.p,
+                             (const double *)nullptr, 0.0, ncons);
+        const double upd = std::sqrt(red.dot(S->du.p, S->du.p, n));
+        if (it == 0) upd0 = upd;
+        upd_prev = upd;
+        if (o.verbose)
+          std::fprintf(stderr, "[lsmf] step %d it %d |R| %.6e CG %d |du| %.6e\n", step + 1, it, res, lin, upd);
+        if (lift)  // constraints.distribute: increment on constrained DoFs
+          hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->du.p, S->d_cons_list.p,
+                             (const double *)S->d_cons_inc.p, 0.0, ncons);
+        hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->delta.p, (const double *)S->du.p, 1.0,
+                           1.0, n);
+      }
+      if (!converged) throw HipError(LSMF_ERR_NOT_CONVERGED, "No convergence in nonlinear solver!");
+      hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->u_n.p, (const double *)S->delta.p, 1.0,
+                         1.0, n);
+      hip_check(hipStreamSynchronize(s), "step sync");
+      st.newton_iterations = it;
+      st.time_total_s = seconds_since(t_step);
+      if (stats) stats[step] = st;
+      ++step;
+      *n_steps = step;
+      t += o.dt;
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,216 @@
+// lsmf_vec.hpp -- vector kernels of the solver: deterministic dot products
+// (wave-shuffle + fixed two-stage reduction), fused CG and Chebyshev updates,
+// precision conversion, constrained-DoF updates and the tensor-product
+// multigrid transfer (one banded 1D operator applied along one direction).
+//
+// Reference counterparts (all in deal.II, called from
+// include/mf_elasticity.h): SolverCG vector updates and inner products
+// (:3049-3061, :3175-3180), PreconditionChebyshev (:2081-2113),
+// MGTransferMatrixFree prolongate / restrict_and_add / interpolate_to_mg
+// (:1917-1942, :2236-2238), copy_to_mg / copy_from_mg (FP64 <-> FP32).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace lsmf {
+
+constexpr int kRedBlocks = 1024;  // fixed grid of the first reduction stage
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// block sum of NV values per thread -> out[blockIdx.x * NV + k]
+template <int NV> __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out) {
+  __shared__ double part[NV][kRedThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) part[k][wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0;
+#pragma unroll
+    for (int w = 0; w < kRedThreads / 64; ++w) s += part[threadIdx.x][w];
+    out[blockIdx.x * NV + threadIdx.x] = s;
+  }
+}
+
+// stage 2: single block sums kRedBlocks partials (fixed order) -> res[k]
+template <int NV> __global__ void k_reduce_final(const double *__restrict__ part, double *__restrict__ res) {
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0;
+  for (int b = threadIdx.x; b < kRedBlocks; b += kRedThreads)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += part[b * NV + k];
+  block_reduce_store<NV>(v, res);  // launched with one block: res[k]
+}
+
+// dot products a.b (and optionally c.d) over n entries
+template <typename TA, typename TB>
+__global__ void __launch_bounds__(kRedThreads) k_dot(const TA *__restrict__ a, const TB *__restrict__ b, size_t n,
+                                                     double *__restrict__ part) {
+  double v[1] = {0.0};
+  for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads)
+    v[0] += double(a[i]) * double(b[i]);
+  block_reduce_store<1>(v, part);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_dot2(const T *__restrict__ a, const T *__restrict__ b,
+                                                      const T *__restrict__ c, const T *__restrict__ d, size_t n,
+                                                      double *__restrict__ part) {
+  double v[2] = {0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads) {
+    v[0] += double(a[i]) * double(b[i]);
+    v[1] += double(c[i]) * double(d[i]);
+  }
+  block_reduce_store<2>(v, part);
+}
+
+// CG: x += alpha p ; r -= alpha Ap ; partial r.r
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_cg_update(T *__restrict__ x, T *__restrict__ r,
+                                                           const T *__restrict__ p, const T *__restrict__ Ap,
+                                                           double alpha, size_t n, double *__restrict__ part) {
+  double v[1] = {0.0};
+  for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads) {
+    x[i] = x[i] + T(alpha) * p[i];
+    const T ri = r[i] - T(alpha) * Ap[i];
+    r[i] = ri;
+    v[0] += double(ri) * double(ri);
+  }
+  block_reduce_store<1>(v, part);
+}
+
+// p = z + beta p
+template <typename T>
+__global__ void k_xpby(T *__restrict__ p, const T *__restrict__ z, double beta, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + T(beta) * p[i];
+}
+
+// y = a x + b y
+template <typename T>
+__global__ void k_axpby(T *__restrict__ y, const T *__restrict__ x, double a, double b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = T(a) * x[i] + T(b) * y[i];
+}
+
+// z = invD .* r
+template <typename T>
+__global__ void k_jacobi(T *__restrict__ z, const T *__restrict__ r, const T *__restrict__ invd, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = invd[i] * r[i];
+}
+
+// t = b - t
+template <typename T> __global__ void k_residual(T *__restrict__ t, const T *__restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] = b[i] - t[i];
+}
+
+// Chebyshev first step: d = f2 invD (b - t) (t == nullptr: b), x = x + d or x = d
+template <typename T>
+__global__ void k_cheb_first(T *__restrict__ x, T *__restrict__ d, const T *__restrict__ b,
+                             const T *__restrict__ t, const T *__restrict__ invd, double f2, int x_is_zero,
+                             size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const T res = t ? b[i] - t[i] : b[i];
+    const T di = T(f2) * invd[i] * res;
+    d[i] = di;
+    x[i] = x_is_zero ? di : x[i] + di;
+  }
+}
+
+// Chebyshev recurrence (deal.II's vector_updates): d = f1 d + f2 invD (b - t); x += d
+template <typename T>
+__global__ void k_cheb_update(T *__restrict__ x, T *__restrict__ d, const T *__restrict__ b,
+                              const T *__restrict__ t, const T *__restrict__ invd, double f1, double f2, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const T di = T(f1) * d[i] + T(f2) * invd[i] * (b[i] - t[i]);
+    d[i] = di;
+    x[i] += di;
+  }
+}
+
+template <typename TI, typename TO>
+__global__ void k_convert(const TI *__restrict__ a, TO *__restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = TO(a[i]);
+}
+
+template <typename T> __global__ void k_fill(T *__restrict__ a, T v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// a[list[k]] = values ? values[k] : v
+template <typename T>
+__global__ void k_set_list(T *__restrict__ a, const int32_t *__restrict__ list, const double *__restrict__ values,
+                           double v, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) a[list[k]] = T(values ? values[k] : v);
+}
+
+// z = x + y (used for u_total = u_n + delta)
+template <typename T>
+__global__ void k_add(T *__restrict__ z, const T *__restrict__ x, const T *__restrict__ y, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = x[i] + y[i];
+}
+
+// deterministic pseudo-random entries in [-1, 1) (SplitMix64 of the index):
+// start vector of the eigenvalue estimate
+template <typename T> __global__ void k_pseudo_random(T *__restrict__ v, size_t n, uint64_t seed) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    uint64_t z = seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    v[i] = T(double(z >> 11) * (2.0 / 9007199254740992.0) - 1.0);
+  }
+}
+
+// One banded 1D operator applied along direction `dir` of a node array of
+// shape (n2, n1, n0, ncomp) (x fastest); out has n_dir replaced by n_out:
+//   out[.., o, ..] = sum_k w[o * W + k] * in[.., start[o] + k, ..]
+// add != 0: out += (used for prolongate-and-add).
+template <typename T>
+__global__ void k_apply_1d(const T *__restrict__ in, T *__restrict__ out, const int32_t *__restrict__ start,
+                           const double *__restrict__ w, int W, int dir, int n0, int n1, int n2, int n_out, int ncomp,
+                           int add) {
+  // output shape
+  const int m0 = dir == 0 ? n_out : n0, m1 = dir == 1 ? n_out : n1, m2 = dir == 2 ? n_out : n2;
+  const size_t total = (size_t)m0 * m1 * m2 * ncomp;
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  const int c = gid % ncomp;
+  size_t rest = gid / ncomp;
+  const int i0 = rest % m0;
+  rest /= m0;
+  const int i1 = rest % m1;
+  const int i2 = (int)(rest / m1);
+  const int o = dir == 0 ? i0 : (dir == 1 ? i1 : i2);
+  const int s = start[o];
+  const size_t stride = dir == 0 ? (size_t)ncomp : (dir == 1 ? (size_t)n0 * ncomp : (size_t)n0 * n1 * ncomp);
+  const int j0 = dir == 0 ? s : i0, j1 = dir == 1 ? s : i1, j2 = dir == 2 ? s : i2;
+  const T *src = in + (((size_t)j2 * n1 + j1) * n0 + j0) * ncomp + c;
+  double acc = 0.0;
+  for (int k = 0; k < W; ++k) acc += w[o * W + k] * double(src[(size_t)k * stride]);
+  out[gid] = add ? T(double(out[gid]) + acc) : T(acc);
+}
+
+}  // namespace lsmf

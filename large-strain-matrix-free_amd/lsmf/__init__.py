@@ -323,3 +323,141 @@ def _hiprt():
 
 
 from .workload import *  # noqa: E402,F401,F403
+
+
+# ---------------------------------------------------------------------------
+# Newton / GMG-CG solver (lsmf_solver_* in include/lsmf.h)
+# ---------------------------------------------------------------------------
+class SolverOptions(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("degree", ctypes.c_int), ("n_q_1d", ctypes.c_int),
+                ("n_cells_coarse", ctypes.c_int * 3), ("n_levels", ctypes.c_int), ("geometry", ctypes.c_int),
+                ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3), ("scale", ctypes.c_double),
+                ("caching", ctypes.c_int), ("mg_precision", ctypes.c_int), ("mg_caching", ctypes.c_int),
+                ("smoother_degree", ctypes.c_int), ("smoothing_range", ctypes.c_double),
+                ("eig_cg_iterations", ctypes.c_int), ("coarse_reduction", ctypes.c_double),
+                ("coarse_eig_iterations", ctypes.c_int), ("tol_lin", ctypes.c_double),
+                ("max_lin_iterations", ctypes.c_int), ("max_newton", ctypes.c_int), ("tol_f", ctypes.c_double),
+                ("tol_u", ctypes.c_double), ("dt", ctypes.c_double), ("t_end", ctypes.c_double),
+                ("dirichlet_components", ctypes.c_uint * 6), ("dirichlet_increment", (ctypes.c_double * 3) * 6),
+                ("neumann_sides", ctypes.c_uint), ("neumann_traction", (ctypes.c_double * 3) * 6),
+                ("verbose", ctypes.c_int)]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [("newton_iterations", ctypes.c_int), ("linear_iterations", ctypes.c_int),
+                ("max_linear_iterations", ctypes.c_int), ("residual_0", ctypes.c_double),
+                ("residual_final", ctypes.c_double), ("time_total_s", ctypes.c_double),
+                ("time_setup_s", ctypes.c_double), ("time_linear_s", ctypes.c_double),
+                ("time_residual_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_ssig = {
+    "lsmf_solver_options_default": (_i, [ctypes.POINTER(SolverOptions)]),
+    "lsmf_solver_create": (_i, [ctypes.POINTER(SolverOptions), _vp, _vp, ctypes.POINTER(_vp)]),
+    "lsmf_solver_destroy": (_i, [_vp]),
+    "lsmf_solver_run": (_i, [_vp, _i, ctypes.POINTER(StepStats), ctypes.POINTER(_i)]),
+    "lsmf_solver_get_solution": (_i, [_vp, _vp]),
+    "lsmf_solver_n_dofs": (ctypes.c_size_t, [_vp, _i]),
+    "lsmf_solver_set_state": (_i, [_vp, _vp]),
+    "lsmf_solver_precondition": (_i, [_vp, _vp, _vp]),
+    "lsmf_solver_cg": (_i, [_vp, _vp, _vp, ctypes.c_double, _i, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
+    "lsmf_solver_level_info": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(_i)]),
+    "lsmf_solver_transfer": (_i, [_vp, _i, _i, _vp, _vp]),
+    "lsmf_solver_fine_operator": (_vp, [_vp]),
+}
+for _name, (_res, _args) in _ssig.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+_sig.update(_ssig)
+EXPORTED = sorted(_sig)
+
+
+def solver_options(**kw):
+    """lsmf_solver_options with the reference defaults, overridden by kw.
+    dirichlet: {side: (components, increments)}, neumann: {side: traction}"""
+    o = SolverOptions()
+    _check(_lib.lsmf_solver_options_default(ctypes.byref(o)))
+    dirichlet = kw.pop("dirichlet", {})
+    neumann = kw.pop("neumann", {})
+    for k, v in kw.items():
+        if k in ("caching", "mg_caching") and isinstance(v, str):
+            v = CACHING[v]
+        if k == "mg_precision" and isinstance(v, str):
+            v = PRECISION[v]
+        if k == "geometry" and isinstance(v, str):
+            v = GEOMETRY[v]
+        if k in ("n_cells_coarse", "lo", "hi"):
+            arr = getattr(o, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(o, k, v)
+    for side, (comps, incs) in dirichlet.items():
+        o.dirichlet_components[side] = sum(1 << c for c in comps)
+        for c, val in zip(comps, incs):
+            o.dirichlet_increment[side][c] = val
+    for side, tr in neumann.items():
+        o.neumann_sides |= 1 << side
+        for c, val in enumerate(tr):
+            o.neumann_traction[side][c] = val
+    return o
+
+
+class Solver:
+    """Load-stepping Newton with FP64 CG preconditioned by a GMG V-cycle."""
+
+    def __init__(self, options, mu, lam):
+        self.options = options
+        self._mu = np.ascontiguousarray(mu, dtype=np.float64)
+        self._lam = np.ascontiguousarray(lam, dtype=np.float64)
+        h = _vp()
+        _check(_lib.lsmf_solver_create(ctypes.byref(options), self._mu.ctypes.data, self._lam.ctypes.data,
+                                       ctypes.byref(h)))
+        self._h = h
+        self.n_levels = options.n_levels
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lsmf_solver_destroy(h)
+            self._h = None
+
+    def n_dofs(self, level=-1):
+        return int(_lib.lsmf_solver_n_dofs(self._h, level))
+
+    def run(self, max_steps=1000):
+        stats = (StepStats * max_steps)()
+        n = ctypes.c_int()
+        _check(_lib.lsmf_solver_run(self._h, max_steps, stats, ctypes.byref(n)))
+        return [stats[k].as_dict() for k in range(n.value)]
+
+    def solution(self):
+        u = np.zeros(self.n_dofs())
+        _check(_lib.lsmf_solver_get_solution(self._h, u.ctypes.data))
+        return u
+
+    def set_state(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        _check(_lib.lsmf_solver_set_state(self._h, u.ctypes.data))
+
+    def precondition(self, z, r):
+        _check(_lib.lsmf_solver_precondition(self._h, _ptr(z), _ptr(r)))
+
+    def cg(self, x, b, tol, max_it=1000):
+        its = ctypes.c_int()
+        res = ctypes.c_double()
+        _check(_lib.lsmf_solver_cg(self._h, _ptr(x), _ptr(b), tol, max_it, ctypes.byref(its), ctypes.byref(res)))
+        return its.value, res.value
+
+    def level_info(self, level):
+        a, b, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _check(_lib.lsmf_solver_level_info(self._h, level, ctypes.byref(a), ctypes.byref(b), ctypes.byref(d)))
+        return a.value, b.value, d.value
+
+    def transfer(self, level, mode, dst, src):
+        _check(_lib.lsmf_solver_transfer(self._h, level, mode, _ptr(dst), _ptr(src)))

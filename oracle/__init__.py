@@ -104,10 +104,11 @@ class OracleOperator:
                                       _p(self.coords), _p(self.constrained), CACHING[caching])
         if not self._h:
             raise ValueError(f"oracle: unsupported (dim, degree, nq) = ({dim}, {degree}, {self.nq})")
+        self._destroy = lib().orc_op_destroy
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().orc_op_destroy(self._h)
+            self._destroy(self._h)
             self._h = None
 
     def set_material(self, mu, lam):

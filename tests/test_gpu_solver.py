@@ -1,0 +1,230 @@
+"""GPU solver stack: multigrid transfer, V-cycle, PCG and the Newton driver.
+
+  * transfer: prolongation reproduces Q_p fields exactly, restriction is its
+    transpose, interpolating a prolongated field returns it (R P = I)
+  * the V-cycle is a symmetric positive definite preconditioner
+  * GPU Newton with GMG on the reference's tests/cook_mf_gmg_form_1.prm
+    (2x2 coarse mesh refined 4 times, Q1, GMG) reproduces the blessed output
+    (Newton counts, corner displacement to 6 digits), with FP32 and FP64
+    level operators
+  * the 2D unit-square configuration (Q2, 5 levels, Chebyshev degree 4, 5
+    prescribed-displacement steps) agrees with the oracle's Newton solve
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import lsmf
+import oracle
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_outputs.json")))
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def tensor_poly(coords, p):
+    """vector field with components in Q_p (degree <= p per direction)"""
+    x = coords
+    dim = x.shape[1]
+    f = np.zeros_like(x)
+    for c in range(dim):
+        v = np.ones(len(x))
+        for d in range(dim):
+            v *= (0.3 + c + d) + 0.7 * x[:, d] ** p - 0.2 * x[:, d]
+        f[:, c] = v
+    return f.reshape(-1)
+
+
+@pytest.mark.parametrize("dim,p", [(2, 1), (2, 2), (3, 2), (3, 4)])
+def test_transfer_exactness(dim, p, device):
+    import torch
+    log(f"[transfer] dim={dim} p={p}")
+    nc = [2, 3, 2][:dim]
+    o = lsmf.solver_options(dim=dim, degree=p, n_q_1d=p + 1, n_cells_coarse=nc, n_levels=3, caching="tensor4",
+                            mg_caching="tensor4", mg_precision="fp64")
+    mesh = lsmf.Mesh(dim, p, [k * 4 for k in nc])
+    S = lsmf.Solver(o, np.ones(mesh.n_cells), np.ones(mesh.n_cells))
+    for level in (1, 2):
+        fine = lsmf.Mesh(dim, p, [k << level for k in nc])
+        coarse = lsmf.Mesh(dim, p, [k << (level - 1) for k in nc])
+        uc = torch.tensor(tensor_poly(coarse.coords, p), device=device)
+        uf = torch.zeros(fine.n_dofs, dtype=torch.float64, device=device)
+        S.transfer(level, 0, uf, uc)
+        assert np.allclose(uf.cpu().numpy(), tensor_poly(fine.coords, p), atol=1e-12)
+        # interpolation of the prolongated field returns the coarse values
+        back = torch.zeros(coarse.n_dofs, dtype=torch.float64, device=device)
+        S.transfer(level, 2, back, uf)
+        assert np.allclose(back.cpu().numpy(), uc.cpu().numpy(), atol=1e-12)
+        # restriction = transpose of prolongation
+        a = torch.rand(coarse.n_dofs, dtype=torch.float64, device=device)
+        b = torch.rand(fine.n_dofs, dtype=torch.float64, device=device)
+        Pa = torch.zeros_like(b)
+        S.transfer(level, 0, Pa, a)
+        PTb = torch.zeros_like(a)
+        S.transfer(level, 1, PTb, b)
+        assert abs(float(Pa @ b) - float(a @ PTb)) < 1e-12 * abs(float(Pa @ b))
+
+
+def small_problem(dim, p, nc, n_levels, prec, caching="tensor4", seed=3):
+    n = [k << (n_levels - 1) for k in nc]
+    mesh = lsmf.Mesh(dim, p, n)
+    mu, lam, _ = lsmf.inclusion_material(mesh.cell_centres(), dim, 1.0, 0.3, 4, 0.2, 10.0, seed)
+    o = lsmf.solver_options(dim=dim, degree=p, n_q_1d=p + 1, n_cells_coarse=nc, n_levels=n_levels, caching=caching,
+                            mg_caching=caching, mg_precision=prec,
+                            dirichlet={0: (list(range(dim)), [0.0] * dim), 1: ([0], [0.01])})
+    return mesh, mu, lam, o
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_vcycle_is_spd_and_cg_converges(prec, device):
+    import torch
+    log(f"[vcycle] {prec}")
+    mesh, mu, lam, o = small_problem(3, 2, [2, 2, 2], 3, prec)
+    S = lsmf.Solver(o, mu, lam)
+    u, _ = lsmf.smooth_displacement(mesh.coords, 0.002, 1)
+    S.set_state(u)
+    for l in range(3):
+        lmax, lmin, deg = S.level_info(l)
+        assert lmax > lmin > 0 and deg >= 1
+        log(f"  level {l}: lambda_max {lmax:.4f} lambda_min {lmin:.4e} degree {deg}")
+    cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
+    mask = torch.tensor(1 - cons, dtype=torch.float64, device=device)
+    g = torch.Generator(device=device)
+    g.manual_seed(1)
+    r = torch.rand(mesh.n_dofs, dtype=torch.float64, device=device, generator=g) * mask
+    w = torch.rand(mesh.n_dofs, dtype=torch.float64, device=device, generator=g) * mask
+    Mr, Mw = torch.zeros_like(r), torch.zeros_like(r)
+    S.precondition(Mr, r)
+    S.precondition(Mw, w)
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    assert abs(float(w @ Mr) - float(r @ Mw)) < tol * abs(float(w @ Mr))
+    assert float(r @ Mr) > 0
+    assert float((Mr * (1 - mask)).abs().max()) == 0.0
+    x = torch.zeros_like(r)
+    its, res = S.cg(x, r, 1e-8 * float(r.norm()))
+    log(f"  PCG its {its} residual {res:.3e}")
+    assert its < 30
+
+
+@pytest.mark.parametrize("dim,nc,L", [(2, [2, 2], 3), (3, [2, 2, 2], 2)])
+def test_vcycle_and_pcg_match_oracle_gmg(dim, nc, L, device):
+    """GPU V-cycle (FP64 levels) = the oracle's restated V-cycle; PCG
+    iteration counts within +-1 (FP64 levels) and +-2 (FP32 levels)"""
+    import torch
+    from oracle.gmg import GMG
+    log(f"[gmg parity] dim={dim} levels={L}")
+    meshes = [lsmf.Mesh(dim, 2, [k << l for k in nc]) for l in range(L)]
+    cons = [m.constrained_mask({0: list(range(dim)), 1: [0]}) for m in meshes]
+    fm = meshes[-1]
+    mu, lam, _ = lsmf.inclusion_material(fm.cell_centres(), dim, 1.0, 0.3, 4, 0.2, 10.0, 3)
+    u, _ = lsmf.smooth_displacement(fm.coords, 0.002, 1)
+    ref = GMG(meshes, cons, mu, lam)
+    ref.setup(u)
+    r = np.random.default_rng(1).random(fm.n_dofs)
+    r[cons[-1] == 1] = 0
+    z_ref = ref.precondition(r)
+    _, its_ref, _ = ref.pcg(ref.levels[-1].op, r, 1e-8 * np.linalg.norm(r))
+    for prec, z_tol, it_tol in [("fp64", 1e-8, 1), ("fp32", 1e-3, 2)]:
+        o = lsmf.solver_options(dim=dim, degree=2, n_q_1d=3, n_cells_coarse=nc, n_levels=L, caching="tensor4",
+                                mg_caching="tensor4", mg_precision=prec,
+                                dirichlet={0: (list(range(dim)), [0.0] * dim), 1: ([0], [0.0])})
+        S = lsmf.Solver(o, mu, lam)
+        S.set_state(u)
+        for l in range(L):
+            lmax, lmin, deg = S.level_info(l)
+            assert abs(lmax - ref.levels[l].lmax_est) < 1e-6 * lmax, (prec, l)
+            if prec == "fp64":
+                assert deg == ref.levels[l].degree
+        rd = torch.tensor(r, device=device)
+        z = torch.zeros_like(rd)
+        S.precondition(z, rd)
+        err = float(np.abs(z.cpu().numpy() - z_ref).max() / np.abs(z_ref).max())
+        log(f"  {prec}: V-cycle rel diff {err:.2e}")
+        assert err < z_tol
+        x = torch.zeros_like(rd)
+        its, _ = S.cg(x, rd, 1e-8 * float(np.linalg.norm(r)))
+        log(f"  {prec}: PCG its {its} oracle {its_ref}")
+        assert abs(its - its_ref) <= it_tol
+
+
+def cook_gmg_options(prec, caching):
+    prm = GOLD["cook_mf_gmg_form_1"]["prm"]
+    mu = float(prm["Shear modulus"])
+    nu = float(prm["Poisson's ratio"])
+    o = lsmf.solver_options(dim=2, degree=1, n_q_1d=2, n_cells_coarse=[2, 2],
+                            n_levels=int(prm["Global refinement"]) + 1, geometry="cook", lo=[0, 0], hi=[48, 44],
+                            scale=float(prm["Grid scale"]), caching=caching, mg_caching=caching, mg_precision=prec,
+                            tol_lin=float(prm["Residual"]), tol_f=float(prm["Tolerance force"]),
+                            tol_u=float(prm["Tolerance displacement"]),
+                            max_newton=int(prm["Max iterations Newton-Raphson"]), dt=float(prm["Time step size"]),
+                            t_end=float(prm["End time"]), dirichlet={0: ([0, 1], [0.0, 0.0])},
+                            neumann={1: [0.0, 1.0 / 16e-6]})
+    return o, mu, 2 * mu * nu / (1 - 2 * nu)
+
+
+@pytest.mark.parametrize("prec,caching", [("fp32", "scalar"), ("fp64", "tensor4"), ("fp32", "tensor2")])
+def test_cook_gmg_newton_matches_blessed_output(prec, caching, device):
+    log(f"[cook gmg newton] {prec} {caching}")
+    out = GOLD["cook_mf_gmg_form_1"]["output"]
+    o, mu, lam = cook_gmg_options(prec, caching)
+    n_cells = 32 * 32
+    S = lsmf.Solver(o, np.full(n_cells, mu), np.full(n_cells, lam))
+    assert S.n_dofs() == out["n_dofs"]
+    stats = S.run()
+    u = S.solution()
+    mesh = lsmf.Mesh(2, 1, [32, 32], lo=[0, 0], hi=[48, 44], geometry="cook", scale=1e-3)
+    corner = int(np.argmin(np.sum((mesh.coords - np.array(out["steps"][0]["point"])) ** 2, axis=1)))
+    assert len(stats) == len(out["steps"])
+    assert stats[-1]["newton_iterations"] == out["steps"][-1]["newton"]
+    for k, st in enumerate(out["steps"]):
+        assert stats[k]["newton_iterations"] == st["newton"], k
+    got = [float(f"{v:.6g}") for v in u[2 * corner:2 * corner + 2]]
+    assert got == out["steps"][-1]["displacement"], (got, out["steps"][-1]["displacement"])
+    log(f"  CG iterations per step: {[s['linear_iterations'] for s in stats]}")
+
+
+def test_cook_gmg_every_step_displacement(device):
+    """displacement history: rerun step by step with t_end = k dt"""
+    out = GOLD["cook_mf_gmg_form_1"]["output"]
+    mesh = lsmf.Mesh(2, 1, [32, 32], lo=[0, 0], hi=[48, 44], geometry="cook", scale=1e-3)
+    corner = int(np.argmin(np.sum((mesh.coords - np.array(out["steps"][0]["point"])) ** 2, axis=1)))
+    o, mu, lam = cook_gmg_options("fp32", "tensor4")
+    for k in (1, 4, 7):
+        o.t_end = 0.1 * k + 1e-9
+        S = lsmf.Solver(o, np.full(1024, mu), np.full(1024, lam))
+        stats = S.run()
+        u = S.solution()
+        assert len(stats) == k
+        got = [float(f"{v:.6g}") for v in u[2 * corner:2 * corner + 2]]
+        assert got == out["steps"][k - 1]["displacement"], (k, got)
+
+
+def test_config0_newton_matches_oracle(device):
+    """2D unit square 32^2 Q2, 8 inclusions, 5 levels, Chebyshev degree 4,
+    x=0 clamped, u_x = 0.01 per step at x=1 over 5 steps"""
+    log("[config0] GPU Newton vs oracle")
+    cfg = lsmf.CONFIGS[0]
+    mesh, mu, lam = lsmf.build_config_mesh(0)
+    o = lsmf.solver_options(dim=2, degree=2, n_q_1d=3, n_cells_coarse=[2, 2], n_levels=cfg["mg_levels"],
+                            caching="tensor4", mg_caching="tensor4", mg_precision="fp32",
+                            smoother_degree=cfg["cheb_degree"], dt=0.2, t_end=1.0, tol_lin=1e-6,
+                            dirichlet={0: ([0, 1], [0.0, 0.0]), 1: ([0], [cfg["increment"]])})
+    S = lsmf.Solver(o, mu, lam)
+    stats = S.run()
+    u = S.solution()
+    assert len(stats) == cfg["load_steps"]
+    cons = mesh.constrained_mask({0: [0, 1], 1: [0]})
+    inc = np.zeros(mesh.n_dofs)
+    inc[mesh.boundary_nodes(1) * 2] = cfg["increment"]
+    ref = oracle.OracleOperator(2, 2, mesh.cell_nodes, mesh.coords, cons, caching="tensor4")
+    ref.set_material(mu, lam)
+    its, _, u_ref = ref.newton(0.2, 1.0, dirichlet_inc=inc, tol_lin=1e-6)
+    log(f"  GPU Newton {[s['newton_iterations'] for s in stats]} oracle {list(its)}")
+    assert all(abs(a["newton_iterations"] - b) <= 1 for a, b in zip(stats, its))
+    assert np.abs(u - u_ref).max() < 1e-7 * np.abs(u_ref).max()
+    assert abs(u[mesh.boundary_nodes(1)[0] * 2] - 0.05) < 1e-12
