@@ -85,6 +85,32 @@ def cpu_baseline(cfg_id, caching, n_sample, seconds):
     }
 
 
+def newton_solve(cfg_id, n_override, mu, lam, precision, tol, verbose=False):
+    """Load-stepping Newton with FP64 CG preconditioned by a GMG V-cycle whose
+    level operators (tangents, Chebyshev smoother, transfer) run in
+    `precision`; u_x prescribed at x = 1 as an increment per load step, x = 0
+    clamped (include/mf_elasticity.h:2932-3007 semantics)."""
+    import numpy as np
+    import lsmf
+
+    cfg = lsmf.CONFIGS[cfg_id]
+    n = cfg["n"] if n_override is None else n_override
+    dim = cfg["dim"]
+    n_levels = int(round(np.log2(n // 2))) + 1
+    steps = cfg["load_steps"]
+    o = lsmf.solver_options(dim=dim, degree=cfg["degree"], n_q_1d=cfg["degree"] + 1, n_cells_coarse=[2] * dim,
+                            n_levels=n_levels, caching=cfg["caching"], mg_caching=cfg["caching"],
+                            mg_precision=precision, smoother_degree=cfg.get("cheb_degree", 5), tol_lin=tol,
+                            tol_f=tol, dt=1.0 / steps, t_end=1.0 + 1e-9, verbose=1 if verbose else 0,
+                            dirichlet={0: (list(range(dim)), [0.0] * dim), 1: ([0], [cfg["increment"]])})
+    t0 = time.perf_counter()
+    S = lsmf.Solver(o, mu, lam)
+    t_create = time.perf_counter() - t0
+    stats = S.run()
+    u = S.solution()
+    return {"create_s": t_create, "stats": stats, "u": u, "n_levels": n_levels}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +123,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-n", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="skip the FP32-GMG vs FP64-GMG Newton solves")
+    ap.add_argument("--solve-tol", type=float, default=1e-8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, run 3 plain device copies of the source vector (known bytes) so "
                          "HBM counters (FETCH_SIZE / WRITE_SIZE) can be scaled")
@@ -170,11 +198,37 @@ def main():
     if args.n is None and os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
 
+    del op, ud, x, y
+    torch.cuda.empty_cache()
+
+    solve = None
+    if not args.no_solve and "load_steps" in cfg:
+        res = {}
+        for prec in ("fp32", "fp64"):
+            progress(f"Newton solve with {prec} GMG levels (tol {args.solve_tol:g})")
+            res[prec] = newton_solve(args.config, args.n, mu, lam, prec, args.solve_tol)
+            st = res[prec]["stats"]
+            progress(f"  {prec}: {sum(s['time_total_s'] for s in st):.2f} s, Newton {[s['newton_iterations'] for s in st]}"
+                     f", CG {[s['linear_iterations'] for s in st]}")
+        t32 = sum(s["time_total_s"] for s in res["fp32"]["stats"])
+        t64 = sum(s["time_total_s"] for s in res["fp64"]["stats"])
+        u32, u64 = res["fp32"]["u"], res["fp64"]["u"]
+        solve = {
+            "fp32_gmg_s": round(t32, 3), "fp64_gmg_s": round(t64, 3), "speedup_fp32_vs_fp64": round(t64 / t32, 3),
+            "tol": args.solve_tol, "load_steps": len(res["fp32"]["stats"]), "mg_levels": res["fp32"]["n_levels"],
+            "newton_iterations": {p: [s["newton_iterations"] for s in res[p]["stats"]] for p in res},
+            "cg_iterations": {p: [s["linear_iterations"] for s in res[p]["stats"]] for p in res},
+            "final_rel_residual": {p: [s["residual_final"] / s["residual_0"] for s in res[p]["stats"]] for p in res},
+            "setup_s": {p: round(sum(s["time_setup_s"] for s in res[p]["stats"]), 3) for p in res},
+            "linear_s": {p: round(sum(s["time_linear_s"] for s in res[p]["stats"]), 3) for p in res},
+            "rel_diff_u_fp32_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+            "timed": "Newton loop of all load steps (residual, caches, level setup, CG); hierarchy creation excluded",
+        }
+        del res
+
     cpu = None
     if not args.no_cpu:
         progress(f"CPU baseline: oracle on {args.cpu_n}^3 sample for ~{args.cpu_seconds:.0f} s")
-        del op
-        torch.cuda.empty_cache()
         cpu = cpu_baseline(args.config, caching, args.cpu_n, args.cpu_seconds)
 
     workload = (f"config {args.config}: {mesh.n_cells_dir[0]}^{mesh.dim} Q{mesh.degree} {caching} "
@@ -201,6 +255,7 @@ def main():
                      "kernel_avg_ms": round(avg_launch_s * 1e3, 5), "launches_per_step": launches_per_vmult,
                      "algorithmic_bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu,
+        "newton_solve": solve,
     }
     print(json.dumps(line), flush=True)
 
