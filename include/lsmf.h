@@ -247,7 +247,14 @@ typedef struct {
   unsigned neumann_sides;            /* bit s: traction on side s */
   double neumann_traction[6][3];     /* traction = t * value, reference configuration */
   int verbose;
+  int coarse_solver;             /* LSMF_COARSE_CHEBYSHEV (reference) or LSMF_COARSE_DIRECT */
+  int use_graph;                 /* replay the V-cycle as a captured HIP graph */
 } lsmf_solver_options;
+
+/* coarse-level solver: the reference's Chebyshev iteration used as a solver
+ * (include/mf_elasticity.h:2087-2096, 2113), or an exact FP64 solve with the
+ * inverse of the assembled coarse matrix (coarsest level up to 4096 DoFs) */
+enum { LSMF_COARSE_CHEBYSHEV = 0, LSMF_COARSE_DIRECT = 1 };
 
 typedef struct {
   int newton_iterations;  /* as printed by "Converged in N Newton iterations" */

@@ -228,6 +228,41 @@ static void tridiag_extremes(const std::vector<double> &a, const std::vector<dou
   lmax = kth((int)n - 1);
 }
 
+// inverse of a symmetric positive definite matrix (row-major, in place) by
+// Cholesky factorisation; returns false if not positive definite
+static bool spd_inverse(std::vector<double> &A, int n) {
+  std::vector<double> Lm(A);
+  for (int j = 0; j < n; ++j) {
+    double d = Lm[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) d -= Lm[(size_t)j * n + k] * Lm[(size_t)j * n + k];
+    if (!(d > 0)) return false;
+    d = std::sqrt(d);
+    Lm[(size_t)j * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double s = Lm[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) s -= Lm[(size_t)i * n + k] * Lm[(size_t)j * n + k];
+      Lm[(size_t)i * n + j] = s / d;
+    }
+  }
+  // inv(L) in place (lower), then A^-1 = inv(L)^T inv(L)
+  std::vector<double> Li((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    Li[(size_t)i * n + i] = 1.0 / Lm[(size_t)i * n + i];
+    for (int j = 0; j < i; ++j) {
+      double s = 0;
+      for (int k = j; k < i; ++k) s += Lm[(size_t)i * n + k] * Li[(size_t)k * n + j];
+      Li[(size_t)i * n + j] = -s / Lm[(size_t)i * n + i];
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double s = 0;
+      for (int k = i; k < n; ++k) s += Li[(size_t)k * n + i] * Li[(size_t)k * n + j];
+      A[(size_t)i * n + j] = A[(size_t)j * n + i] = s;
+    }
+  return true;
+}
+
 struct GMGBase {
   virtual ~GMGBase() {}
   virtual void setup(const double *u_fine, double *tmp64a, double *tmp64b) = 0;
@@ -317,7 +352,24 @@ template <typename T> class GMG : public GMGBase {
         tmax64_ = std::max(tmax64_, tmax);
       }
     }
+    if (o.coarse_solver == LSMF_COARSE_DIRECT) {
+      const LevelGeom &g0 = *geoms[0];
+      if (g0.n_dofs > 4096)
+        throw HipError(LSMF_ERR_INVALID_ARGUMENT, "direct coarse solve needs a coarsest level with <= 4096 DoFs");
+      if constexpr (std::is_same<T, double>::value) {
+        coarse64_ = lv_[0].op;
+      } else {
+        coarse64_own_.reset(new NeoHookOperatorHIP<double>());
+        lsmf_mesh_desc md{o.dim, o.degree, g0.mesh->n_cells, g0.mesh->n_nodes, g0.mesh->cell_nodes, g0.mesh->coords,
+                          g0.cons.data(), g0.mesh->cell_colour};
+        coarse64_own_->initialize(md, o.n_q_1d, o.mg_caching, s);
+        coarse64_own_->set_material(g0.mu.data(), g0.lam.data());
+        coarse64_ = coarse64_own_.get();
+      }
+      ainv_.resize(g0.n_dofs * g0.n_dofs);
+    }
   }
+  ~GMG() override { drop_graph(); }
 
   int n_levels() const override { return (int)lv_.size(); }
   size_t level_dofs(int l) const override { return lv_[l].g->n_dofs; }
@@ -364,6 +416,7 @@ template <typename T> class GMG : public GMGBase {
   // (include/mf_elasticity.h:1926-1942), level caches and diagonals
   // (:2055-2073), Chebyshev data (:2081-2113)
   void setup(const double *u_fine, double *ta, double *tb) override {
+    drop_graph();
     const int L = (int)lv_.size();
     const double *ucur = u_fine;
     for (int l = L - 1; l >= 0; --l) {
@@ -379,7 +432,9 @@ template <typename T> class GMG : public GMGBase {
         v.op->cache();
       }
       v.op->compute_diagonal();
-      if (l == 0) {
+      if (l == 0 && opt_.coarse_solver == LSMF_COARSE_DIRECT) {
+        assemble_coarse_inverse(ucur);
+      } else if (l == 0) {
         const int its = opt_.coarse_eig_iterations > 0 ? opt_.coarse_eig_iterations : (int)v.g->n_dofs;
         estimate_eigenvalues(v, its);
         const double lmax = 1.2 * v.lmax_est;
@@ -405,6 +460,28 @@ template <typename T> class GMG : public GMGBase {
   // z = M r, the V-cycle preconditioner between FP64 vectors of the finest
   // level (copy_to_mg / copy_from_mg convert FP64 <-> level precision)
   void precondition(double *z, const double *r) override {
+    if (!opt_.use_graph) {
+      precondition_eager(z, r);
+      return;
+    }
+    // the V-cycle is a fixed sequence of launches between two setups:
+    // capture it once and replay it (launch overhead of the small levels)
+    if (!graph_exec_ || z != graph_z_ || r != graph_r_) {
+      drop_graph();
+      hipGraph_t g = nullptr;
+      hip_check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "begin capture");
+      precondition_eager(z, r);
+      hip_check(hipStreamEndCapture(s_, &g), "end capture");
+      hip_check(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0), "graph instantiate");
+      (void)hipGraphDestroy(g);
+      graph_z_ = z;
+      graph_r_ = r;
+    }
+    hip_check(hipGraphLaunch(graph_exec_, s_), "graph launch");
+  }
+
+ private:
+  void precondition_eager(double *z, const double *r) {
     const int L = (int)lv_.size() - 1;
     Level<T> &f = lv_[L];
     const size_t n = f.g->n_dofs;
@@ -413,7 +490,43 @@ template <typename T> class GMG : public GMGBase {
     hipLaunchKernelGGL((k_convert<T, double>), dim3(nblk(n)), dim3(256), 0, s_, f.sol.p, z, n);
   }
 
- private:
+  void drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+    graph_z_ = nullptr;
+    graph_r_ = nullptr;
+  }
+
+  // FP64 coarse matrix by probing unit vectors, inverted on the host
+  void assemble_coarse_inverse(const double *u0) {
+    Level<T> &v = lv_[0];
+    const int n = (int)v.g->n_dofs;
+    if (coarse64_own_) {
+      coarse64_->set_linearization(u0);
+      coarse64_->cache();
+    }
+    DeviceArray<double> e(n), col((size_t)n * n);
+    for (int j = 0; j < n; ++j) {
+      hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, s_, e.p, 0.0, (size_t)n);
+      hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(1), 0, s_, e.p + j, 1.0, (size_t)1);
+      coarse64_->vmult(col.p + (size_t)j * n, e.p);
+    }
+    std::vector<double> A((size_t)n * n);
+    hip_check(hipMemcpyAsync(A.data(), col.p, A.size() * sizeof(double), hipMemcpyDeviceToHost, s_), "coarse matrix");
+    hip_check(hipStreamSynchronize(s_), "coarse matrix");
+    // symmetrise (round-off) and invert
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < i; ++j) {
+        const double a = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
+        A[(size_t)i * n + j] = A[(size_t)j * n + i] = a;
+      }
+    if (!spd_inverse(A, n)) throw HipError(LSMF_ERR_NOT_CONVERGED, "coarse matrix is not positive definite");
+    ainv_.upload(A.data(), A.size());
+    v.degree = 0;
+    v.lmax_est = v.lmin_est = 0.0;
+  }
+
+
   // Chebyshev smoother / coarse solver on level v: x from zero (vmult) or
   // from its current value (step)
   void chebyshev(Level<T> &v, T *x, const T *b, bool x_zero) {
@@ -445,7 +558,11 @@ template <typename T> class GMG : public GMGBase {
     Level<T> &v = lv_[l];
     const size_t n = v.g->n_dofs;
     if (l == 0) {
-      chebyshev(v, v.sol.p, v.rhs.p, true);
+      if (opt_.coarse_solver == LSMF_COARSE_DIRECT)
+        hipLaunchKernelGGL(k_dense_matvec<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s_,
+                           (const double *)ainv_.p, (const T *)v.rhs.p, v.sol.p, (int)n);
+      else
+        chebyshev(v, v.sol.p, v.rhs.p, true);
       return;
     }
     chebyshev(v, v.sol.p, v.rhs.p, true);  // pre-smoothing from zero
@@ -508,6 +625,12 @@ template <typename T> class GMG : public GMGBase {
   Reducer red_;
   std::vector<Level<T>> lv_;
   size_t tmax64_ = 0;
+  NeoHookOperatorHIP<double> *coarse64_ = nullptr;
+  std::unique_ptr<NeoHookOperatorHIP<double>> coarse64_own_;
+  DeviceArray<double> ainv_;
+  hipGraphExec_t graph_exec_ = nullptr;
+  double *graph_z_ = nullptr;
+  const double *graph_r_ = nullptr;
 };
 
 }  // namespace lsmf
@@ -731,6 +854,8 @@ lsmf_status lsmf_solver_options_default(lsmf_solver_options *o) {
   o->tol_u = 1e-6;
   o->dt = 0.1;
   o->t_end = 1.0;
+  o->coarse_solver = LSMF_COARSE_CHEBYSHEV;
+  o->use_graph = 1;
   return LSMF_OK;
 }
 

@@ -184,6 +184,19 @@ template <typename T> __global__ void k_pseudo_random(T *__restrict__ v, size_t 
   }
 }
 
+// dense FP64 matrix-vector product x = M b (coarse direct solve with the
+// inverse coarse matrix); one wave per row, shuffle reduction
+template <typename T>
+__global__ void k_dense_matvec(const double *__restrict__ M, const T *__restrict__ b, T *__restrict__ x, int n) {
+  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += M[(size_t)row * n + j] * double(b[j]);
+  s = wave_sum(s);
+  if (lane == 0) x[row] = T(s);
+}
+
 // One banded 1D operator applied along direction `dir` of a node array of
 // shape (n2, n1, n0, ncomp) (x fastest); out has n_dir replaced by n_out:
 //   out[.., o, ..] = sum_k w[o * W + k] * in[.., start[o] + k, ..]

@@ -340,7 +340,10 @@ class SolverOptions(ctypes.Structure):
                 ("tol_u", ctypes.c_double), ("dt", ctypes.c_double), ("t_end", ctypes.c_double),
                 ("dirichlet_components", ctypes.c_uint * 6), ("dirichlet_increment", (ctypes.c_double * 3) * 6),
                 ("neumann_sides", ctypes.c_uint), ("neumann_traction", (ctypes.c_double * 3) * 6),
-                ("verbose", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("coarse_solver", ctypes.c_int), ("use_graph", ctypes.c_int)]
+
+
+COARSE = {"chebyshev": 0, "direct": 1}
 
 
 class StepStats(ctypes.Structure):
@@ -391,6 +394,8 @@ def solver_options(**kw):
             v = PRECISION[v]
         if k == "geometry" and isinstance(v, str):
             v = GEOMETRY[v]
+        if k == "coarse_solver" and isinstance(v, str):
+            v = COARSE[v]
         if k in ("n_cells_coarse", "lo", "hi"):
             arr = getattr(o, k)
             for i, x in enumerate(v):

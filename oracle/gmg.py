@@ -116,11 +116,12 @@ class GMG:
     """levels: list of (mesh, constrained mask) coarsest .. finest"""
 
     def __init__(self, meshes, cons, mu, lam, caching="tensor4", smoother_degree=5, smoothing_range=20.0,
-                 eig_its=30, coarse_reduction=1e-3, coarse_eig_its=0):
+                 eig_its=30, coarse_reduction=1e-3, coarse_eig_its=0, coarse_solver="chebyshev"):
         self.dim = meshes[-1].dim
         self.p = meshes[-1].degree
         self.opts = dict(smoother_degree=smoother_degree, smoothing_range=smoothing_range, eig_its=eig_its,
                          coarse_reduction=coarse_reduction, coarse_eig_its=coarse_eig_its)
+        self.coarse_solver = coarse_solver
         self.levels = []
         mus, lams = [np.asarray(mu, float)], [np.asarray(lam, float)]
         for l in range(len(meshes) - 1, 0, -1):
@@ -148,6 +149,12 @@ class GMG:
             lv.op.set_linearization(us[l])
             lv.op.cache()
             lv.invd = 1.0 / lv.op.diagonal()
+            if l == 0 and self.coarse_solver == "direct":
+                n0 = lv.mesh.n_dofs
+                A0 = np.column_stack([lv.op.vmult(np.eye(n0)[:, i]) for i in range(n0)])
+                lv.Ainv = np.linalg.inv(0.5 * (A0 + A0.T))
+                lv.degree, lv.lmax_est, lv.lmin_est = 0, 0.0, 0.0
+                continue
             if l == 0:
                 its = self.opts["coarse_eig_its"] or lv.mesh.n_dofs
                 lv.lmax_est, lv.lmin_est = self.eigen(lv, its)
@@ -224,6 +231,8 @@ class GMG:
     def vcycle(self, l, b):
         lv = self.levels[l]
         if l == 0:
+            if self.coarse_solver == "direct":
+                return lv.Ainv @ b
             return self.chebyshev(lv, None, b, True)
         x = self.chebyshev(lv, None, b, True)
         t = b - lv.op.vmult(x)

@@ -152,6 +152,53 @@ def test_vcycle_and_pcg_match_oracle_gmg(dim, nc, L, device):
         assert abs(its - its_ref) <= it_tol
 
 
+def test_direct_coarse_solve_and_graph_replays(device):
+    """exact FP64 coarse solve (assembled coarse matrix) matches the oracle's
+    V-cycle with a dense coarse solve; the captured V-cycle graph gives the
+    eager result on every replay, with eager work in between"""
+    import torch
+    from oracle.gmg import GMG
+    dim, nc, L = 3, [2, 2, 2], 3
+    meshes = [lsmf.Mesh(dim, 2, [k << l for k in nc]) for l in range(L)]
+    cons = [m.constrained_mask({0: [0, 1, 2], 1: [0]}) for m in meshes]
+    fm = meshes[-1]
+    mu, lam, _ = lsmf.inclusion_material(fm.cell_centres(), dim, 1.0, 0.3, 4, 0.2, 10.0, 3)
+    u, _ = lsmf.smooth_displacement(fm.coords, 0.002, 1)
+    ref = GMG(meshes, cons, mu, lam, coarse_solver="direct")
+    ref.setup(u)
+    r = np.random.default_rng(1).random(fm.n_dofs)
+    r[cons[-1] == 1] = 0
+    z_ref = ref.precondition(r)
+    _, its_ref, _ = ref.pcg(ref.levels[-1].op, r, 1e-8 * np.linalg.norm(r))
+    rd = torch.tensor(r, device=device)
+    out = {}
+    for use_graph in (0, 1):
+        o = lsmf.solver_options(dim=dim, degree=2, n_q_1d=3, n_cells_coarse=nc, n_levels=L, caching="tensor4",
+                                mg_caching="tensor4", mg_precision="fp64", coarse_solver="direct",
+                                use_graph=use_graph, dirichlet={0: ([0, 1, 2], [0.0] * 3), 1: ([0], [0.0])})
+        S = lsmf.Solver(o, mu, lam)
+        S.set_state(u)
+        zs = []
+        for rep in range(4):
+            z = torch.zeros_like(rd)
+            S.precondition(z, rd)
+            zs.append(z.cpu().numpy())
+            junk = torch.rand(1000, device=device).sum()  # eager work between replays
+            torch.cuda.synchronize()
+            del junk
+        for z in zs:
+            assert np.array_equal(z, zs[0])
+        err = float(np.abs(zs[0] - z_ref).max() / np.abs(z_ref).max())
+        log(f"  graph={use_graph}: V-cycle rel diff vs oracle {err:.2e}")
+        assert err < 1e-8
+        x = torch.zeros_like(rd)
+        its, _ = S.cg(x, rd, 1e-8 * float(np.linalg.norm(r)))
+        log(f"  graph={use_graph}: PCG its {its} oracle {its_ref}")
+        assert abs(its - its_ref) <= 1
+        out[use_graph] = zs[0]
+    assert np.array_equal(out[0], out[1])
+
+
 def cook_gmg_options(prec, caching):
     prm = GOLD["cook_mf_gmg_form_1"]["prm"]
     mu = float(prm["Shear modulus"])
