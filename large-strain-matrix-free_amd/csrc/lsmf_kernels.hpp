@@ -506,18 +506,29 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   T *B = A + NB * NQP;
   T *Xc = Xall + (slot < CPW ? slot : 0) * NC * DIM;
 
-  // per-qp cache loads first: they stay in flight through the SF stages
+  // Load order matters: vmcnt is an in-order counter, so consuming a load
+  // waits for every load issued before it.  The dependent index -> source
+  // gather goes first; the per-qp cache stream is issued after it and stays
+  // in flight through the sum-factorization stages (LDS-only barriers).
+  // The loads are branch-free (idle lanes read a valid cell of this colour
+  // and discard the values) so the wait counts stay static.
+  const size_t cell_s = active ? (size_t)cell : (size_t)P.cell_begin;
+  const int r_s = on ? r : 0;
+  const uint32_t code = P.idx[cell_s * NQP + r_s];
+  const size_t node = code & kNodeMask;
+  T sv[DIM];
+  if constexpr (!KIN) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) sv[c] = P.src[node * DIM + c];
+  }
+  __builtin_amdgcn_sched_barrier(0);
   T cch[NF > 0 ? NF : 1];
   if constexpr ((PH & PH_QD) && NF > 0 && VAR != V_RESIDUAL) {
-    if (active) {
-      const T *cp = P.cache + (size_t)cell * NQP + r;
+    const T *cp = P.cache + cell_s * NQP + r_s;
 #pragma unroll
-      for (int f = 0; f < NF; ++f) cch[f] = cp[(size_t)f * P.fs];
-    }
+    for (int f = 0; f < NF; ++f) cch[f] = cp[(size_t)f * P.fs];
   }
-  uint32_t code = 0;
-  if (active) code = P.idx[(size_t)cell * NQP + r];
-  const size_t node = code & kNodeMask;
+  __builtin_amdgcn_sched_barrier(0);
 
   T F[DIM][DIM], Jri[DIM][DIM], detJr = T(1);
   if constexpr (KIN && (PH & PH_QD)) {
@@ -531,7 +542,9 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
     if (active) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
-        T v = P.src[node * DIM + c];
+        T v;
+        if constexpr (KIN) v = P.src[node * DIM + c];
+        else v = sv[c];
         if (CONSTR && ((code >> (kConsShift + c)) & 1u)) v = T(0);
         A[c * NQP + r] = v;
       }
@@ -616,14 +629,20 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
       SF<DIM, N, T>::integrate(tab, A, B, on, r, i0, i1, i2, Phi, out);
   }
 
-  // colour-ordered scatter (distribute_local_to_global, :769-780)
+  // colour-ordered scatter (distribute_local_to_global, :769-780): all reads
+  // of the node's values are issued before the writes (one round trip)
   if (active) {
+    const bool first = !ADD && (code & kFirstBit);
+    T old[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      const bool skip = CONSTR && ((code >> (kConsShift + c)) & 1u);
+      old[c] = (first || skip) ? T(0) : P.dst[node * DIM + c];
+    }
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
       if (CONSTR && ((code >> (kConsShift + c)) & 1u)) continue;
-      T *d = P.dst + node * DIM + c;
-      if (!ADD && (code & kFirstBit)) *d = out[c];
-      else *d += out[c];
+      P.dst[node * DIM + c] = old[c] + out[c];
     }
   }
 }
