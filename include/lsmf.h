@@ -168,6 +168,18 @@ size_t lsmf_memory_consumption(const lsmf_op *op);
  * be FP64. */
 lsmf_status lsmf_residual(lsmf_op *op, double *rhs_dev, lsmf_stream stream);
 
+/* Vector-Jacobian product of lsmf_residual (new; the reference has no
+ * adjoint -- its nearest relative differentiates P(F) per quadrature point
+ * with Sacado in include/mf_ad_nh_operator.h:415-485).  For the cotangent w
+ * (device FP64, n_nodes*dim):
+ *   du_dev      = (dR/du)^T w = -K w            (K: unconstrained tangent)
+ *   dmu[c]      = (dR/dmu_c)^T w,  dlambda[c] = (dR/dlambda_c)^T w
+ * with dmu, dlambda device FP64 arrays [n_cells] in mesh cell order.
+ * Kinematics (F, J, ln J) are recomputed from the linearisation state, no
+ * cache is read or stored.  FP64 operators only. */
+lsmf_status lsmf_residual_vjp(lsmf_op *op, const double *w_dev, double *du_dev, double *dmu, double *dlambda,
+                              lsmf_stream stream);
+
 /* thread-local description of the last error on this thread */
 const char *lsmf_last_error(void);
 
@@ -249,7 +261,12 @@ typedef struct {
   int verbose;
   int coarse_solver;             /* LSMF_COARSE_CHEBYSHEV (reference) or LSMF_COARSE_DIRECT */
   int use_graph;                 /* replay the V-cycle as a captured HIP graph */
+  int preconditioner;            /* LSMF_PRECONDITIONER_GMG / _JACOBI / _NONE ("Preconditioner type") */
+  double preconditioner_relaxation; /* Jacobi damping omega ("Preconditioner relaxation") */
+  double tol_f_abs, tol_u_abs;   /* "Absolute tolerance force" / "... displacement" (:2262-2266) */
 } lsmf_solver_options;
+
+enum { LSMF_PRECONDITIONER_GMG = 0, LSMF_PRECONDITIONER_JACOBI = 1, LSMF_PRECONDITIONER_NONE = 2 };
 
 /* coarse-level solver: the reference's Chebyshev iteration used as a solver
  * (include/mf_elasticity.h:2087-2096, 2113), or an exact FP64 solve with the
@@ -274,6 +291,18 @@ lsmf_status lsmf_solver_create(const lsmf_solver_options *opt, const double *mu,
 lsmf_status lsmf_solver_destroy(lsmf_solver *s);
 /* load stepping t = dt, 2 dt, ... <= t_end; stats[k] per step */
 lsmf_status lsmf_solver_run(lsmf_solver *s, int max_steps, lsmf_step_stats *stats, int *n_steps);
+/* one load step (the next t = t_prev + dt); *done = 1 and no work when the
+ * next t exceeds t_end; *t_out receives the step's time */
+lsmf_status lsmf_solver_step(lsmf_solver *s, lsmf_step_stats *stats, double *t_out, int *done);
+/* Dirichlet increments of the next load steps (the values interpolate_boundary_values
+ * puts into the constraints at Newton iteration 0, include/mf_elasticity.h:2963-2990):
+ * host array over all finest DoFs, entries of unconstrained DoFs ignored. Replaces
+ * the per-side constants of the options. */
+lsmf_status lsmf_solver_set_dirichlet_increment(lsmf_solver *s, const double *inc_host);
+/* surface traction per side and component (reference configuration) for the next
+ * load steps, applied as given (the options' traction is scaled by t instead);
+ * the Neumann term of assemble_system (include/mf_elasticity.h:2859-2892) */
+lsmf_status lsmf_solver_set_traction(lsmf_solver *s, const double traction[6][3]);
 lsmf_status lsmf_solver_get_solution(const lsmf_solver *s, double *u_host);
 size_t lsmf_solver_n_dofs(const lsmf_solver *s, int level); /* level -1 = finest */
 /* linearise the fine operator and every level at u (host FP64, finest numbering) */

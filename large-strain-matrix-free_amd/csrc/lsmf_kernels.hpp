@@ -38,7 +38,17 @@ constexpr uint32_t kNodeMask = (1u << 28) - 1u;
 constexpr int kConsShift = 28;  // bits 28..30: component c constrained
 constexpr uint32_t kFirstBit = 1u << 31;  // this cell writes (not adds) first
 
-enum Variant { V_SCALAR = 0, V_TENSOR2 = 1, V_TENSOR4 = 2, V_TENSOR4_NS = 3, V_SCALAR_REF = 4, V_RESIDUAL = 100 };
+// V_RESIDUAL: FP64 internal forces; V_ADJOINT: vector-Jacobian product of
+// the residual with respect to u and the per-cell mu, lambda
+enum Variant {
+  V_SCALAR = 0,
+  V_TENSOR2 = 1,
+  V_TENSOR4 = 2,
+  V_TENSOR4_NS = 3,
+  V_SCALAR_REF = 4,
+  V_RESIDUAL = 100,
+  V_ADJOINT = 101
+};
 
 // phase bits (reference MFMask, include/mf_nh_operator.h:73-86)
 constexpr unsigned PH_SF = 1u << 3;
@@ -443,6 +453,10 @@ template <typename Number> struct VmultParams {
   const Number *__restrict__ mu;      // per sorted cell
   const Number *__restrict__ lambda;
   int cell_begin, cell_end;
+  // adjoint: per-cell outputs in mesh order (perm: sorted -> mesh cell)
+  double *__restrict__ dmu;
+  double *__restrict__ dlambda;
+  const int32_t *__restrict__ perm;
 };
 
 // Kinematic prologue shared by the scalar vmult and the residual: gathers u
@@ -486,12 +500,15 @@ template <int DIM, int N, typename T, int VAR, bool ADD, bool CONSTR, unsigned P
 __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   using CS = CellShape<DIM, N>;
   constexpr int NQP = CS::NQP, CPW = CS::CPW, NB = CS::NB, NC = CS::NC;
-  constexpr int NF = CacheFields<DIM, (VAR == V_RESIDUAL ? V_SCALAR : VAR)>::value;
-  constexpr bool KIN = (VAR == V_SCALAR || VAR == V_RESIDUAL);
+  constexpr bool RECOMP = (VAR == V_RESIDUAL || VAR == V_ADJOINT);  // no cache, c from ln J in FP64
+  constexpr int NF = CacheFields<DIM, (RECOMP ? V_SCALAR : VAR)>::value;
+  constexpr bool KIN = (VAR == V_SCALAR || RECOMP);
   constexpr int TABN = 4 * N * N + 2 * N;
-  __shared__ T lds[CPW * 2 * NB * NQP + TABN + (KIN ? CPW * NC * DIM : 0)];
+  constexpr int RED = (VAR == V_ADJOINT) ? CPW * 2 * NQP : 0;  // per-cell reductions
+  __shared__ T lds[CPW * 2 * NB * NQP + TABN + (KIN ? CPW * NC * DIM : 0) + RED];
   T *tab = lds + CPW * 2 * NB * NQP;
   T *Xall = tab + TABN;
+  T *Rall = Xall + (KIN ? CPW * NC * DIM : 0);
 
   const int t = threadIdx.x;
   if (t < TABN) tab[t] = P.tab[t];
@@ -523,7 +540,7 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   }
   __builtin_amdgcn_sched_barrier(0);
   T cch[NF > 0 ? NF : 1];
-  if constexpr ((PH & PH_QD) && NF > 0 && VAR != V_RESIDUAL) {
+  if constexpr ((PH & PH_QD) && NF > 0 && !RECOMP) {
     const T *cp = P.cache + cell_s * NQP + r_s;
 #pragma unroll
     for (int f = 0; f < NF; ++f) cch[f] = cp[(size_t)f * P.fs];
@@ -582,7 +599,9 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
       T K[DIM][DIM];  // J_cur^-1 = J_ref^-1 F^-1
       grad_times<DIM>(Jri, Fi, K);
       T tau[DIM][DIM];
-      const T cs = (VAR == V_RESIDUAL) ? (m - T(2) * l * T(log(double(J)))) : (active ? cch[0] : T(0));
+      T cs;
+      if constexpr (RECOMP) cs = m - T(2) * l * T(log(double(J)));
+      else cs = active ? cch[0] : T(0);
 #pragma unroll
       for (int i = 0; i < DIM; ++i)
 #pragma unroll
@@ -621,12 +640,51 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
             Q[i][j] = s * dw;
           }
         flux_times_kt<DIM>(Q, K, Phi);
+        if constexpr (VAR == V_ADJOINT) {
+          // (dR/du)^T w = -K w, and with tau = mu (b - I) + 2 lambda ln J I:
+          // (dR/dmu_c)^T w = -sum_q (b - I) : g dw, (dR/dlambda_c)^T w =
+          // -sum_q 2 ln J tr(g) dw, where g = grad_x w.  Recomputed from u.
+#pragma unroll
+          for (int c = 0; c < DIM; ++c)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) Phi[c][d] = -Phi[c][d];
+          T bg = T(0);
+#pragma unroll
+          for (int i = 0; i < DIM; ++i)
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) {
+              // b_ij - delta_ij = (tau_ij + cs delta_ij) / mu - delta_ij
+              T s = F[i][0] * F[j][0];
+#pragma unroll
+              for (int k = 1; k < DIM; ++k) s = fma(F[i][k], F[j][k], s);
+              bg = fma(s - (i == j ? T(1) : T(0)), g[i][j], bg);
+            }
+          if (on) {
+            T *Rs = Rall + (slot < CPW ? slot : 0) * 2 * NQP;
+            Rs[r] = -bg * dw;
+            Rs[NQP + r] = -T(2) * T(log(double(J))) * tr * dw;
+          }
+        }
       }
     }
     if constexpr (DIM == 3)
       SF<DIM, N, T>::integrate(tab, B, A, on, r, i0, i1, i2, Phi, out);
     else
       SF<DIM, N, T>::integrate(tab, A, B, on, r, i0, i1, i2, Phi, out);
+    if constexpr (VAR == V_ADJOINT) {
+      // the integration's barriers ordered the reduction writes above
+      if (active && r == 0) {
+        const T *Rs = Rall + slot * 2 * NQP;
+        T a = T(0), b = T(0);
+        for (int q = 0; q < NQP; ++q) {
+          a += Rs[q];
+          b += Rs[NQP + q];
+        }
+        const int mc = P.perm[cell];
+        P.dmu[mc] = double(a);
+        P.dlambda[mc] = double(b);
+      }
+    }
   }
 
   // colour-ordered scatter (distribute_local_to_global, :769-780): all reads

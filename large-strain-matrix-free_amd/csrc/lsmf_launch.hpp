@@ -23,6 +23,9 @@ template <int DIM, int N> struct Launch {
     P.lambda = static_cast<const T *>(op->d_lambda);
     P.cell_begin = op->colour_ptr[k];
     P.cell_end = op->colour_ptr[k + 1];
+    P.dmu = op->adj_dmu;
+    P.dlambda = op->adj_dlambda;
+    P.perm = op->d_perm;
     return P;
   }
 
@@ -155,8 +158,15 @@ template <int DIM, int N> struct Launch {
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
 
+  static int residual_vjp(const lsmf_op *op, double *du, const double *w, hipStream_t s) {
+    if (op->precision != LSMF_FP64) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual VJP needs an FP64 operator");
+    constexpr unsigned FULL = PH_SF | PH_QD;
+    run_colours<double, V_ADJOINT, false, false, FULL>(op, du, w, s);
+    return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
+  }
+
   static const LaunchTable *table() {
-    static const LaunchTable t{&vmult, &cache, &diagonal, &residual};
+    static const LaunchTable t{&vmult, &cache, &diagonal, &residual, &residual_vjp};
     return &t;
   }
 };

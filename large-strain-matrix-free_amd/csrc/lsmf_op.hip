@@ -77,7 +77,7 @@ template <typename T> int dalloc(T **p, size_t n, const char *what) {
 void free_op(lsmf_op *op) {
   void *ptrs[] = {op->d_idx,  op->d_coords,    op->d_tab,  op->d_tab64, op->d_mu64,     op->d_lambda64,
                   op->d_mu,   op->d_lambda,    op->d_cache, op->d_diag, op->d_inv_diag, op->d_cons,
-                  op->d_bad};
+                  op->d_bad,  op->d_perm};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (op->d_u64) (void)hipFree(op->d_u64);
@@ -242,7 +242,8 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       (rc = dalloc((char **)&op->d_cache, (size_t)op->n_fields * op->field_stride * es, "cache")) ||
       (rc = dalloc((char **)&op->d_diag, op->n_dofs * es, "diagonal")) ||
       (rc = dalloc((char **)&op->d_inv_diag, op->n_dofs * es, "inverse diagonal")) ||
-      (rc = dalloc(&op->d_cons, cons.size(), "constrained list")) || (rc = dalloc(&op->d_bad, 1, "counter"))) {
+      (rc = dalloc(&op->d_cons, cons.size(), "constrained list")) || (rc = dalloc(&op->d_bad, 1, "counter")) ||
+      (rc = dalloc(&op->d_perm, (size_t)nc, "cell permutation"))) {
     free_op(op);
     delete op;
     return rc;
@@ -270,6 +271,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   }
   if (e == hipSuccess && !cons.empty())
     e = hipMemcpy(op->d_cons, cons.data(), cons.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(op->d_perm, op->perm.data(), (size_t)nc * sizeof(int32_t), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     free_op(op);
     delete op;
@@ -427,6 +429,21 @@ lsmf_status lsmf_profile_read(lsmf_op *op, double *total_ms, int *n_launches) {
   *total_ms = sum;
   *n_launches = (int)op->prof->used;
   op->prof->used = 0;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_residual_vjp(lsmf_op *op, const double *w_dev, double *du_dev, double *dmu, double *dlambda,
+                              lsmf_stream stream) {
+  if (!op || !w_dev || !du_dev || !dmu || !dlambda) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual_vjp: null argument");
+  if (op->precision != LSMF_FP64) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual_vjp: operator must be FP64");
+  if ((const void *)w_dev == (const void *)du_dev) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual_vjp: w and du alias");
+  if (!op->material_set || !op->linearization_set)
+    return fail(LSMF_ERR_INVALID_ARGUMENT, "residual_vjp: material and linearisation must be set");
+  op->adj_dmu = dmu;
+  op->adj_dlambda = dlambda;
+  const int rc = launch_table(op->dim, op->N)->residual_vjp(op, du_dev, w_dev, (hipStream_t)stream);
+  op->adj_dmu = op->adj_dlambda = nullptr;
+  if (rc) return fail(rc, "residual_vjp: kernel launch failed");
   return LSMF_OK;
 }
 

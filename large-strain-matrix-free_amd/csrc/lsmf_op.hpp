@@ -70,6 +70,8 @@ struct lsmf_op {
   int32_t *d_cons = nullptr;        // constrained DoF list
   int n_cons = 0;
   int *d_bad = nullptr;
+  int32_t *d_perm = nullptr;        // sorted position -> mesh cell
+  double *adj_dmu = nullptr, *adj_dlambda = nullptr;  // outputs of the running residual VJP
   bool material_set = false, linearization_set = false, cached = false, diag_ok = false;
   size_t bytes_idx = 0, bytes_cache = 0;
 };
@@ -84,6 +86,7 @@ struct LaunchTable {
   int (*cache)(const lsmf_op *, hipStream_t);
   int (*diagonal)(const lsmf_op *, hipStream_t);
   int (*residual)(const lsmf_op *, double *rhs, hipStream_t);
+  int (*residual_vjp)(const lsmf_op *, double *du, const double *w, hipStream_t);
 };
 
 const LaunchTable *launch_table(int dim, int N);

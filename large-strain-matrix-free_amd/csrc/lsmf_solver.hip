@@ -651,6 +651,9 @@ struct lsmf_solver {
   DeviceArray<int32_t> d_cons_list;
   DeviceArray<double> d_cons_inc;
   bool has_increment = false, has_neumann = false;
+  bool neumann_absolute = false;  // traction set for the step (not scaled by t)
+  double t_prev = 0.0;  // time of the last completed load step
+  int n_steps_done = 0;
   ~lsmf_solver() {
     mg.reset();
     fine.clear();
@@ -696,15 +699,16 @@ void coarsen_material(int dim, const int *nc_fine, const std::vector<double> &f,
 }
 
 // Neumann load for t = 1 on the finest mesh (reference configuration)
-std::vector<double> neumann_vector(const lsmf_solver_options &o, const LevelGeom &g) {
+std::vector<double> neumann_vector(const lsmf_solver_options &o, const double (&traction)[6][3], unsigned sides,
+                                   const LevelGeom &g) {
   std::vector<double> f(g.n_dofs, 0.0);
-  if (!o.neumann_sides) return f;
+  if (!sides) return f;
   Tables1D t;
   t.init(o.degree, o.n_q_1d);
   const int dim = o.dim, p = o.degree, N = p + 1;
   const int npc = dim == 3 ? N * N * N : N * N;
   for (int s = 0; s < 2 * dim; ++s) {
-    if (!((o.neumann_sides >> s) & 1u)) continue;
+    if (!((sides >> s) & 1u)) continue;
     const int cnt = lsmf_mesh_boundary_faces(g.mesh, s, nullptr, nullptr);
     std::vector<int32_t> cells(cnt), faces(cnt);
     lsmf_mesh_boundary_faces(g.mesh, s, cells.data(), faces.data());
@@ -763,7 +767,7 @@ std::vector<double> neumann_vector(const lsmf_solver_options &o, const LevelGeom
             phi *= lagrange_value(t.nodes, ai % N, xi[d]);
             ai /= N;
           }
-          for (int c = 0; c < dim; ++c) f[(size_t)cn[a] * dim + c] += phi * o.neumann_traction[s][c] * jxw;
+          for (int c = 0; c < dim; ++c) f[(size_t)cn[a] * dim + c] += phi * traction[s][c] * jxw;
         }
       }
     }
@@ -784,7 +788,30 @@ int catch_all(const std::function<void()> &fn) {
   }
 }
 
-// FP64 PCG with the V-cycle (SolverCG::solve with PreconditionMG)
+// PreconditionMG (V-cycle), PreconditionJacobi(omega) from the fine
+// operator's inverse diagonal, or PreconditionIdentity
+// (include/mf_elasticity.h:3125-3170)
+void apply_preconditioner(lsmf_solver *S, double *z, const double *r) {
+  switch (S->opt.preconditioner) {
+    case LSMF_PRECONDITIONER_GMG:
+      S->mg->precondition(z, r);
+      break;
+    case LSMF_PRECONDITIONER_JACOBI:
+      S->fine.precondition_Jacobi(z, r, S->opt.preconditioner_relaxation);
+      break;
+    default:
+      hip_check(hipMemcpyAsync(z, r, S->n * sizeof(double), hipMemcpyDeviceToDevice, S->s), "identity");
+  }
+}
+
+void setup_preconditioner(lsmf_solver *S) {
+  if (S->opt.preconditioner == LSMF_PRECONDITIONER_GMG)
+    S->mg->setup(S->u_tot.p, S->ta.p, S->tb.p);
+  else if (S->opt.preconditioner == LSMF_PRECONDITIONER_JACOBI)
+    S->fine.compute_diagonal();
+}
+
+// FP64 PCG (SolverCG::solve with the chosen preconditioner)
 int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, double *res_out) {
   const size_t n = S->n;
   hipStream_t s = S->s;
@@ -796,7 +823,7 @@ int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, doub
     *res_out = rn;
     return 0;
   }
-  S->mg->precondition(S->z.p, S->r.p);
+  apply_preconditioner(S, S->z.p, S->r.p);
   hip_check(hipMemcpyAsync(S->p.p, S->z.p, n * sizeof(double), hipMemcpyDeviceToDevice, s), "pcg copy");
   double rz = red.dot(S->r.p, S->z.p, n);
   int it = 0;
@@ -808,7 +835,7 @@ int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, doub
     const double rr = red.cg_update(x, S->r.p, (const double *)S->p.p, (const double *)S->Ap.p, alpha, n);
     rn = std::sqrt(rr);
     if (rn <= tol) break;
-    S->mg->precondition(S->z.p, S->r.p);
+    apply_preconditioner(S, S->z.p, S->r.p);
     const double rz_new = red.dot(S->r.p, S->z.p, n);
     const double beta = rz_new / rz;
     rz = rz_new;
@@ -818,10 +845,10 @@ int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, doub
   return rn <= tol ? it : -it;
 }
 
-void set_state(lsmf_solver *S, const double *u_dev) {
-  S->fine.set_linearization(u_dev);
+void set_state(lsmf_solver *S) {
+  S->fine.set_linearization(S->u_tot.p);
   S->fine.cache();
-  S->mg->setup(u_dev, S->ta.p, S->tb.p);
+  setup_preconditioner(S);
 }
 
 }  // namespace
@@ -856,6 +883,8 @@ lsmf_status lsmf_solver_options_default(lsmf_solver_options *o) {
   o->t_end = 1.0;
   o->coarse_solver = LSMF_COARSE_CHEBYSHEV;
   o->use_graph = 1;
+  o->preconditioner = LSMF_PRECONDITIONER_GMG;
+  o->preconditioner_relaxation = 0.65;
   return LSMF_OK;
 }
 
@@ -866,6 +895,9 @@ lsmf_status lsmf_solver_create(const lsmf_solver_options *o, const double *mu, c
     return fail(LSMF_ERR_INVALID_ARGUMENT, "solver_create: bad dim or n_levels");
   if (o->mg_precision != LSMF_FP32 && o->mg_precision != LSMF_FP64)
     return fail(LSMF_ERR_INVALID_ARGUMENT, "solver_create: bad mg_precision");
+  if (o->preconditioner < LSMF_PRECONDITIONER_GMG || o->preconditioner > LSMF_PRECONDITIONER_NONE)
+    return fail(LSMF_ERR_INVALID_ARGUMENT, "solver_create: bad preconditioner");
+  if (o->dt <= 0.0) return fail(LSMF_ERR_INVALID_ARGUMENT, "solver_create: dt must be positive");
   *out = nullptr;
   lsmf_solver *S = new (std::nothrow) lsmf_solver;
   if (!S) return fail(LSMF_ERR_OUT_OF_MEMORY, "solver_create: host allocation");
@@ -890,7 +922,9 @@ lsmf_status lsmf_solver_create(const lsmf_solver_options *o, const double *mu, c
                       fg.cons.data(), fg.mesh->cell_colour};
     S->fine.initialize(md, o->n_q_1d, o->caching, S->s);
     S->fine.set_material(fg.mu.data(), fg.lam.data());
-    if (o->mg_precision == LSMF_FP32)
+    if (o->preconditioner != LSMF_PRECONDITIONER_GMG)
+      ;  // no level operators
+    else if (o->mg_precision == LSMF_FP32)
       S->mg.reset(new GMG<float>(*o, S->geoms, &S->fine, S->s));
     else
       S->mg.reset(new GMG<double>(*o, S->geoms, &S->fine, S->s));
@@ -928,7 +962,7 @@ lsmf_status lsmf_solver_create(const lsmf_solver_options *o, const double *mu, c
       S->d_cons_list.upload(S->cons_list.data(), S->cons_list.size());
       S->d_cons_inc.upload(S->cons_inc.data(), S->cons_inc.size());
     }
-    std::vector<double> f = neumann_vector(*o, fg);
+    std::vector<double> f = neumann_vector(*o, o->neumann_traction, o->neumann_sides, fg);
     S->neumann.upload(f.data(), n);
     S->has_neumann = o->neumann_sides != 0;
   });
@@ -958,7 +992,7 @@ lsmf_status lsmf_solver_set_state(lsmf_solver *S, const double *u_host) {
   if (!S || !u_host) return fail(LSMF_ERR_INVALID_ARGUMENT, "set_state: null argument");
   return catch_all([&] {
     S->u_tot.upload(u_host, S->n);
-    set_state(S, S->u_tot.p);
+    set_state(S);
     hip_check(hipStreamSynchronize(S->s), "set_state");
   });
 }
@@ -966,7 +1000,7 @@ lsmf_status lsmf_solver_set_state(lsmf_solver *S, const double *u_host) {
 lsmf_status lsmf_solver_precondition(lsmf_solver *S, double *z, const double *r) {
   if (!S || !z || !r) return fail(LSMF_ERR_INVALID_ARGUMENT, "precondition: null argument");
   return catch_all([&] {
-    S->mg->precondition(z, r);
+    apply_preconditioner(S, z, r);
     hip_check(hipStreamSynchronize(S->s), "precondition");
   });
 }
@@ -982,14 +1016,14 @@ lsmf_status lsmf_solver_cg(lsmf_solver *S, double *x, const double *b, double to
 }
 
 lsmf_status lsmf_solver_level_info(const lsmf_solver *S, int level, double *lmax, double *lmin, int *degree) {
-  if (!S || !lmax || !lmin || !degree || level < 0 || level >= S->mg->n_levels())
+  if (!S || !S->mg || !lmax || !lmin || !degree || level < 0 || level >= S->mg->n_levels())
     return fail(LSMF_ERR_INVALID_ARGUMENT, "level_info: bad argument");
   S->mg->info(level, *lmax, *lmin, *degree);
   return LSMF_OK;
 }
 
 lsmf_status lsmf_solver_transfer(lsmf_solver *S, int level, int mode, double *dst, const double *src) {
-  if (!S || !dst || !src || level < 1 || level >= (int)S->geoms.size() || mode < 0 || mode > 2)
+  if (!S || !S->mg || !dst || !src || level < 1 || level >= (int)S->geoms.size() || mode < 0 || mode > 2)
     return fail(LSMF_ERR_INVALID_ARGUMENT, "transfer: bad argument");
   return catch_all([&] { S->mg->transfer64(level, mode, dst, src); });
 }
@@ -1002,103 +1036,157 @@ lsmf_status lsmf_solver_get_solution(const lsmf_solver *S, double *u_host) {
   });
 }
 
-// Solid::run with solve_nonlinear_timestep (include/mf_elasticity.h:1187-1222, 2289-2559)
+// solve_nonlinear_timestep (include/mf_elasticity.h:2289-2559) for the
+// load step at time t
+static void newton_step(lsmf_solver *S, double t, int step, lsmf_step_stats &st) {
+  const lsmf_solver_options &o = S->opt;
+  const size_t n = S->n;
+  hipStream_t s = S->s;
+  Reducer &red = *S->red;
+  const int ncons = (int)S->cons_list.size();
+  const auto t_step = Clock::now();
+  st = lsmf_step_stats{};
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, s, S->delta.p, 0.0, n);
+  double res0 = 0, upd0 = 0, upd_prev = 0;
+  bool converged = false;
+  int it = 0;
+  for (; it < o.max_newton; ++it) {
+    auto t0 = Clock::now();
+    // set_total_solution + cache at the new linearisation point
+    hipLaunchKernelGGL(k_add<double>, dim3(nblk(n)), dim3(256), 0, s, S->u_tot.p, (const double *)S->u_n.p,
+                       (const double *)S->delta.p, n);
+    S->fine.set_linearization(S->u_tot.p);
+    S->fine.cache();
+    st.time_setup_s += seconds_since(t0);
+    t0 = Clock::now();
+    // assemble_system: residual + Neumann + lifting (-K g) at it 0
+    abi_check(lsmf_residual(S->fine.handle(), S->rhs.p, s), "residual");
+    if (S->has_neumann)
+      hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->rhs.p, (const double *)S->neumann.p,
+                         S->neumann_absolute ? 1.0 : t, 1.0, n);
+    const bool lift = (it == 0 && S->has_increment);
+    if (lift) {
+      hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, s, S->g.p, 0.0, n);
+      hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->g.p, S->d_cons_list.p,
+                         (const double *)S->d_cons_inc.p, 0.0, ncons);
+      S->fine.vmult(S->Ag.p, S->g.p, LSMF_MASK_PLAIN);
+      hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->rhs.p, (const double *)S->Ag.p, -1.0,
+                         1.0, n);
+    }
+    if (ncons)
+      hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->rhs.p, S->d_cons_list.p,
+                         (const double *)nullptr, 0.0, ncons);
+    const double res = std::sqrt(red.dot(S->rhs.p, S->rhs.p, n));
+    st.time_residual_s += seconds_since(t0);
+    if (it == 0) {
+      res0 = res;
+      st.residual_0 = res;
+    }
+    st.residual_final = res;
+    // check_convergence (include/mf_elasticity.h:2245-2284)
+    if (it > 2) {
+      const double rn = res0 > 0 ? res / res0 : res;
+      const double un = upd0 > 0 ? upd_prev / upd0 : upd_prev;
+      if (res <= o.tol_f_abs || upd_prev <= o.tol_u_abs || (rn <= o.tol_f && un <= o.tol_u)) converged = true;
+    }
+    if (converged) break;
+    t0 = Clock::now();
+    setup_preconditioner(S);
+    hip_check(hipStreamSynchronize(s), "setup sync");
+    st.time_setup_s += seconds_since(t0);
+    t0 = Clock::now();
+    double lin_res = 0;
+    const int lin = pcg(S, S->du.p, S->rhs.p, o.tol_lin * res, o.max_lin_iterations, &lin_res);
+    st.time_linear_s += seconds_since(t0);
+    if (lin < 0) throw HipError(LSMF_ERR_NOT_CONVERGED, "linear solver did not converge");
+    st.linear_iterations += lin;
+    st.max_linear_iterations = std::max(st.max_linear_iterations, lin);
+    if (ncons)
+      hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->du.p, S->d_cons_list.p,
+                         (const double *)nullptr, 0.0, ncons);
+    const double upd = std::sqrt(red.dot(S->du.p, S->du.p, n));
+    if (it == 0) upd0 = upd;
+    upd_prev = upd;
+    if (o.verbose)
+      std::fprintf(stderr, "[lsmf] step %d it %d |R| %.6e CG %d |du| %.6e\n", step + 1, it, res, lin, upd);
+    if (lift)  // constraints.distribute: increment on constrained DoFs
+      hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->du.p, S->d_cons_list.p,
+                         (const double *)S->d_cons_inc.p, 0.0, ncons);
+    hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->delta.p, (const double *)S->du.p, 1.0, 1.0,
+                       n);
+  }
+  if (!converged) throw HipError(LSMF_ERR_NOT_CONVERGED, "No convergence in nonlinear solver!");
+  hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->u_n.p, (const double *)S->delta.p, 1.0, 1.0,
+                     n);
+  hip_check(hipStreamSynchronize(s), "step sync");
+  st.newton_iterations = it;
+  st.time_total_s = seconds_since(t_step);
+}
+
+lsmf_status lsmf_solver_step(lsmf_solver *S, lsmf_step_stats *stats, double *t_out, int *done) {
+  if (!S || !done) return fail(LSMF_ERR_INVALID_ARGUMENT, "step: null argument");
+  *done = 0;
+  return catch_all([&] {
+    // Time::increment accumulates dt (include/mf_elasticity.h:614-653)
+    const double t = S->t_prev + S->opt.dt;
+    if (!(t <= S->opt.t_end)) {
+      *done = 1;
+      return;
+    }
+    lsmf_step_stats st{};
+    newton_step(S, t, S->n_steps_done, st);
+    S->t_prev = t;
+    ++S->n_steps_done;
+    if (stats) *stats = st;
+    if (t_out) *t_out = t;
+  });
+}
+
+lsmf_status lsmf_solver_set_dirichlet_increment(lsmf_solver *S, const double *inc) {
+  if (!S || !inc) return fail(LSMF_ERR_INVALID_ARGUMENT, "set_dirichlet_increment: null argument");
+  return catch_all([&] {
+    S->has_increment = false;
+    for (size_t k = 0; k < S->cons_list.size(); ++k) {
+      S->cons_inc[k] = inc[S->cons_list[k]];
+      if (S->cons_inc[k] != 0.0) S->has_increment = true;
+    }
+    if (!S->cons_list.empty()) {
+      hip_check(hipStreamSynchronize(S->s), "sync");
+      S->d_cons_inc.upload(S->cons_inc.data(), S->cons_inc.size());
+    }
+  });
+}
+
+lsmf_status lsmf_solver_set_traction(lsmf_solver *S, const double traction[6][3]) {
+  if (!S || !traction) return fail(LSMF_ERR_INVALID_ARGUMENT, "set_traction: null argument");
+  return catch_all([&] {
+    double tr[6][3] = {};
+    unsigned sides = 0;
+    for (int s = 0; s < 2 * S->opt.dim; ++s)
+      for (int c = 0; c < S->opt.dim; ++c) {
+        tr[s][c] = traction[s][c];
+        if (tr[s][c] != 0.0) sides |= 1u << s;
+      }
+    std::vector<double> f = neumann_vector(S->opt, tr, sides, *S->geoms.back());
+    hip_check(hipStreamSynchronize(S->s), "sync");
+    S->neumann.upload(f.data(), S->n);
+    S->has_neumann = sides != 0;
+    S->neumann_absolute = true;
+  });
+}
+
+// Solid::run (include/mf_elasticity.h:1187-1222)
 lsmf_status lsmf_solver_run(lsmf_solver *S, int max_steps, lsmf_step_stats *stats, int *n_steps) {
   if (!S || !n_steps) return fail(LSMF_ERR_INVALID_ARGUMENT, "run: null argument");
   *n_steps = 0;
-  return catch_all([&] {
-    const lsmf_solver_options &o = S->opt;
-    const size_t n = S->n;
-    hipStream_t s = S->s;
-    Reducer &red = *S->red;
-    const int ncons = (int)S->cons_list.size();
-    double t = 0.0;
-    t += o.dt;
-    int step = 0;
-    while (t <= o.t_end && step < max_steps) {
-      const auto t_step = Clock::now();
-      lsmf_step_stats st{};
-      hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, s, S->delta.p, 0.0, n);
-      double res0 = 0, upd0 = 0, upd_prev = 0;
-      bool converged = false;
-      int it = 0;
-      for (; it < o.max_newton; ++it) {
-        auto t0 = Clock::now();
-        // set_total_solution + cache at the new linearisation point
-        hipLaunchKernelGGL(k_add<double>, dim3(nblk(n)), dim3(256), 0, s, S->u_tot.p, (const double *)S->u_n.p,
-                           (const double *)S->delta.p, n);
-        S->fine.set_linearization(S->u_tot.p);
-        S->fine.cache();
-        st.time_setup_s += seconds_since(t0);
-        t0 = Clock::now();
-        // assemble_system: residual + Neumann + lifting (-K g) at it 0
-        abi_check(lsmf_residual(S->fine.handle(), S->rhs.p, s), "residual");
-        if (S->has_neumann)
-          hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->rhs.p, (const double *)S->neumann.p,
-                             t, 1.0, n);
-        const bool lift = (it == 0 && S->has_increment);
-        if (lift) {
-          hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, s, S->g.p, 0.0, n);
-          hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->g.p, S->d_cons_list.p,
-                             (const double *)S->d_cons_inc.p, 0.0, ncons);
-          S->fine.vmult(S->Ag.p, S->g.p, LSMF_MASK_PLAIN);
-          hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->rhs.p, (const double *)S->Ag.p, -1.0,
-                             1.0, n);
-        }
-        if (ncons)
-          hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->rhs.p, S->d_cons_list.p,
-                             (const double *)nullptr, 0.0, ncons);
-        const double res = std::sqrt(red.dot(S->rhs.p, S->rhs.p, n));
-        st.time_residual_s += seconds_since(t0);
-        if (it == 0) {
-          res0 = res;
-          st.residual_0 = res;
-        }
-        st.residual_final = res;
-        // check_convergence (include/mf_elasticity.h:2245-2284)
-        if (it > 2) {
-          const double rn = res0 > 0 ? res / res0 : res;
-          const double un = upd0 > 0 ? upd_prev / upd0 : upd_prev;
-          if (res <= 0.0 || upd_prev <= 0.0 || (rn <= o.tol_f && un <= o.tol_u)) converged = true;
-        }
-        if (converged) break;
-        t0 = Clock::now();
-        S->mg->setup(S->u_tot.p, S->ta.p, S->tb.p);
-        hip_check(hipStreamSynchronize(s), "setup sync");
-        st.time_setup_s += seconds_since(t0);
-        t0 = Clock::now();
-        double lin_res = 0;
-        const int lin = pcg(S, S->du.p, S->rhs.p, o.tol_lin * res, o.max_lin_iterations, &lin_res);
-        st.time_linear_s += seconds_since(t0);
-        if (lin < 0) throw HipError(LSMF_ERR_NOT_CONVERGED, "linear solver did not converge");
-        st.linear_iterations += lin;
-        st.max_linear_iterations = std::max(st.max_linear_iterations, lin);
-        if (ncons)
-          hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->du.p, S->d_cons_list.p,
-                             (const double *)nullptr, 0.0, ncons);
-        const double upd = std::sqrt(red.dot(S->du.p, S->du.p, n));
-        if (it == 0) upd0 = upd;
-        upd_prev = upd;
-        if (o.verbose)
-          std::fprintf(stderr, "[lsmf] step %d it %d |R| %.6e CG %d |du| %.6e\n", step + 1, it, res, lin, upd);
-        if (lift)  // constraints.distribute: increment on constrained DoFs
-          hipLaunchKernelGGL(k_set_list<double>, dim3(nblk(ncons)), dim3(256), 0, s, S->du.p, S->d_cons_list.p,
-                             (const double *)S->d_cons_inc.p, 0.0, ncons);
-        hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->delta.p, (const double *)S->du.p, 1.0,
-                           1.0, n);
-      }
-      if (!converged) throw HipError(LSMF_ERR_NOT_CONVERGED, "No convergence in nonlinear solver!");
-      hipLaunchKernelGGL(k_axpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->u_n.p, (const double *)S->delta.p, 1.0,
-                         1.0, n);
-      hip_check(hipStreamSynchronize(s), "step sync");
-      st.newton_iterations = it;
-      st.time_total_s = seconds_since(t_step);
-      if (stats) stats[step] = st;
-      ++step;
-      *n_steps = step;
-      t += o.dt;
-    }
-  });
+  for (int k = 0; k < max_steps; ++k) {
+    int done = 0;
+    const int rc = lsmf_solver_step(S, stats ? stats + k : nullptr, nullptr, &done);
+    if (rc) return rc;
+    if (done) break;
+    ++*n_steps;
+  }
+  return LSMF_OK;
 }
 
 }  // extern "C"

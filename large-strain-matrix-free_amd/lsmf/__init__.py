@@ -93,6 +93,7 @@ _sig = {
     "lsmf_op_size": (ctypes.c_size_t, [_vp]),
     "lsmf_memory_consumption": (ctypes.c_size_t, [_vp]),
     "lsmf_residual": (_i, [_vp, _vp, _vp]),
+    "lsmf_residual_vjp": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "lsmf_last_error": (ctypes.c_char_p, []),
     "lsmf_profile_enable": (_i, [_vp, _i]),
     "lsmf_profile_read": (_i, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
@@ -309,6 +310,13 @@ class NeoHookOperator:
         assert rhs.dtype == torch.float64 and rhs.numel() == self.m()
         _check(_lib.lsmf_residual(self._h, _ptr(rhs), _stream(stream)))
 
+    def residual_vjp(self, w, du, dmu, dlam, stream=None):
+        """du = (dR/du)^T w, dmu = (dR/dmu)^T w, dlam = (dR/dlambda)^T w (device FP64)"""
+        import torch
+        for t, n in ((w, self.m()), (du, self.m()), (dmu, self.n_cells), (dlam, self.n_cells)):
+            assert t.dtype == torch.float64 and t.is_cuda and t.numel() == n and t.is_contiguous()
+        _check(_lib.lsmf_residual_vjp(self._h, _ptr(w), _ptr(du), _ptr(dmu), _ptr(dlam), _stream(stream)))
+
 
 _HIPRT = None
 
@@ -340,10 +348,13 @@ class SolverOptions(ctypes.Structure):
                 ("tol_u", ctypes.c_double), ("dt", ctypes.c_double), ("t_end", ctypes.c_double),
                 ("dirichlet_components", ctypes.c_uint * 6), ("dirichlet_increment", (ctypes.c_double * 3) * 6),
                 ("neumann_sides", ctypes.c_uint), ("neumann_traction", (ctypes.c_double * 3) * 6),
-                ("verbose", ctypes.c_int), ("coarse_solver", ctypes.c_int), ("use_graph", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("coarse_solver", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("preconditioner", ctypes.c_int), ("preconditioner_relaxation", ctypes.c_double),
+                ("tol_f_abs", ctypes.c_double), ("tol_u_abs", ctypes.c_double)]
 
 
 COARSE = {"chebyshev": 0, "direct": 1}
+PRECONDITIONER = {"gmg": 0, "jacobi": 1, "none": 2}
 
 
 class StepStats(ctypes.Structure):
@@ -362,6 +373,9 @@ _ssig = {
     "lsmf_solver_create": (_i, [ctypes.POINTER(SolverOptions), _vp, _vp, ctypes.POINTER(_vp)]),
     "lsmf_solver_destroy": (_i, [_vp]),
     "lsmf_solver_run": (_i, [_vp, _i, ctypes.POINTER(StepStats), ctypes.POINTER(_i)]),
+    "lsmf_solver_step": (_i, [_vp, ctypes.POINTER(StepStats), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
+    "lsmf_solver_set_dirichlet_increment": (_i, [_vp, _vp]),
+    "lsmf_solver_set_traction": (_i, [_vp, ctypes.POINTER((ctypes.c_double * 3) * 6)]),
     "lsmf_solver_get_solution": (_i, [_vp, _vp]),
     "lsmf_solver_n_dofs": (ctypes.c_size_t, [_vp, _i]),
     "lsmf_solver_set_state": (_i, [_vp, _vp]),
@@ -396,6 +410,8 @@ def solver_options(**kw):
             v = GEOMETRY[v]
         if k == "coarse_solver" and isinstance(v, str):
             v = COARSE[v]
+        if k == "preconditioner" and isinstance(v, str):
+            v = PRECONDITIONER[v]
         if k in ("n_cells_coarse", "lo", "hi"):
             arr = getattr(o, k)
             for i, x in enumerate(v):
@@ -440,6 +456,29 @@ class Solver:
         n = ctypes.c_int()
         _check(_lib.lsmf_solver_run(self._h, max_steps, stats, ctypes.byref(n)))
         return [stats[k].as_dict() for k in range(n.value)]
+
+    def step(self):
+        """one load step; None once the next time exceeds t_end, else
+        (t, stats)"""
+        st = StepStats()
+        t = ctypes.c_double()
+        done = ctypes.c_int()
+        _check(_lib.lsmf_solver_step(self._h, ctypes.byref(st), ctypes.byref(t), ctypes.byref(done)))
+        return None if done.value else (t.value, st.as_dict())
+
+    def set_dirichlet_increment(self, inc):
+        """per-DoF increments (host, all finest DoFs) for the next steps"""
+        inc = np.ascontiguousarray(inc, dtype=np.float64)
+        assert inc.shape == (self.n_dofs(),)
+        _check(_lib.lsmf_solver_set_dirichlet_increment(self._h, inc.ctypes.data))
+
+    def set_traction(self, traction):
+        """{side: vector} surface tractions applied as given (not scaled by t)"""
+        tr = ((ctypes.c_double * 3) * 6)()
+        for side, v in traction.items():
+            for c, x in enumerate(v):
+                tr[side][c] = x
+        _check(_lib.lsmf_solver_set_traction(self._h, ctypes.byref(tr)))
 
     def solution(self):
         u = np.zeros(self.n_dofs())

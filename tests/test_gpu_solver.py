@@ -275,3 +275,68 @@ def test_config0_newton_matches_oracle(device):
     assert all(abs(a["newton_iterations"] - b) <= 1 for a, b in zip(stats, its))
     assert np.abs(u - u_ref).max() < 1e-7 * np.abs(u_ref).max()
     assert abs(u[mesh.boundary_nodes(1)[0] * 2] - 0.05) < 1e-12
+
+
+def test_cook_jacobi_newton_matches_blessed_output_per_step(device):
+    """tests/cook_mf_form_1.prm: 32x32 Q1 Cook membrane, MF_CG with
+    PreconditionJacobi(0.65); driven one load step at a time through
+    lsmf_solver_step, every step's Newton count and corner displacement
+    against the blessed output"""
+    log("[cook jacobi] step by step")
+    gold = GOLD["cook_mf_form_1"]
+    prm, out = gold["prm"], gold["output"]
+    mu = float(prm["Shear modulus"])
+    nu = float(prm["Poisson's ratio"])
+    lam = 2 * mu * nu / (1 - 2 * nu)
+    o = lsmf.solver_options(dim=2, degree=1, n_q_1d=2, n_cells_coarse=[32, 32], n_levels=1, geometry="cook",
+                            lo=[0, 0], hi=[48, 44], scale=float(prm["Grid scale"]), caching="tensor4",
+                            preconditioner="jacobi", tol_lin=float(prm["Residual"]),
+                            max_lin_iterations=out["n_dofs"], tol_f=float(prm["Tolerance force"]),
+                            tol_u=float(prm["Tolerance displacement"]),
+                            max_newton=int(prm["Max iterations Newton-Raphson"]), dt=float(prm["Time step size"]),
+                            t_end=float(prm["End time"]), dirichlet={0: ([0, 1], [0.0, 0.0])},
+                            neumann={1: [0.0, 1.0 / 16e-6]})
+    assert abs(o.preconditioner_relaxation - 0.65) < 1e-15
+    S = lsmf.Solver(o, np.full(1024, mu), np.full(1024, lam))
+    assert S.n_dofs() == out["n_dofs"]
+    mesh = lsmf.Mesh(2, 1, [32, 32], lo=[0, 0], hi=[48, 44], geometry="cook", scale=1e-3)
+    corner = int(np.argmin(np.sum((mesh.coords - np.array(out["steps"][0]["point"])) ** 2, axis=1)))
+    k = 0
+    while True:
+        r = S.step()
+        if r is None:
+            break
+        t, st = r
+        ref = out["steps"][k]
+        assert abs(t - ref["time"]) < 1e-12
+        assert st["newton_iterations"] == ref["newton"], k
+        u = S.solution()
+        got = [float(f"{v:.6g}") for v in u[2 * corner:2 * corner + 2]]
+        assert got == ref["displacement"], (k, got, ref["displacement"])
+        k += 1
+    assert k == len(out["steps"])
+    assert S.step() is None
+
+
+@pytest.mark.parametrize("pre", ["none", "jacobi"])
+def test_unpreconditioned_and_jacobi_newton_agree_with_gmg(pre, device):
+    """same 3D Q2 problem solved with GMG, Jacobi and no preconditioner:
+    identical Newton counts, solutions within the linear tolerance"""
+    nc, L = [2, 2, 2], 2
+    mesh = lsmf.Mesh(3, 2, [k << (L - 1) for k in nc])
+    mu, lam = lsmf.lognormal_material(mesh.n_cells, 1.0, 0.3, 0.5, 5)
+
+    def solve(p, levels):
+        o = lsmf.solver_options(dim=3, degree=2, n_q_1d=3, n_cells_coarse=[k << (L - levels) for k in nc],
+                                n_levels=levels, caching="tensor4", mg_precision="fp64", preconditioner=p,
+                                tol_lin=1e-10, max_lin_iterations=5000, dt=0.5, t_end=1.0,
+                                dirichlet={0: ([0, 1, 2], [0.0] * 3), 1: ([0], [0.02])})
+        S = lsmf.Solver(o, mu, lam)
+        return S.run(), S.solution()
+
+    st_g, u_g = solve("gmg", L)
+    st_p, u_p = solve(pre, 1)
+    log(f"  {pre}: Newton {[s['newton_iterations'] for s in st_p]} CG {[s['linear_iterations'] for s in st_p]};"
+        f" GMG CG {[s['linear_iterations'] for s in st_g]}")
+    assert [s["newton_iterations"] for s in st_p] == [s["newton_iterations"] for s in st_g]
+    assert np.abs(u_p - u_g).max() < 1e-8 * np.abs(u_g).max()
