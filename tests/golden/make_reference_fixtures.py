@@ -6,11 +6,14 @@ written by Solid::run / print_solution, include/mf_elasticity.h:1187-1222,
 2615-2690) and writes tests/golden/reference_outputs.json.  Only data is
 kept: the settings of the prm files that define the problem, and the
 numbers of the output files.  Run in the build container; the JSON is the
-committed fixture.
+committed fixture.  The parameter files and blessed outputs themselves (input
+and expected-output data of those tests) are copied to tests/golden/prm/ so
+the parameter-file driver can be run on them unchanged.
 """
 import json
 import os
 import re
+import shutil
 
 REF = "/root/reference/tests"
 NAMES = ["cook_mf_form_1", "cook_mat_form_1", "cook_mf_gmg_form_1", "cook_mf_form_1_tensor4_ns",
@@ -62,6 +65,12 @@ def main():
         if os.path.exists(prm) and os.path.exists(out):
             data[n] = {"prm": parse_prm(prm), "output": parse_output(out)}
     here = os.path.dirname(os.path.abspath(__file__))
+    os.makedirs(os.path.join(here, "prm"), exist_ok=True)
+    for n in NAMES:
+        for ext in (".prm", ".output"):
+            src = os.path.join(REF, n + ext)
+            if os.path.exists(src):
+                shutil.copyfile(src, os.path.join(here, "prm", n + ext))
     with open(os.path.join(here, "reference_outputs.json"), "w") as f:
         json.dump(data, f, indent=1, sort_keys=True)
     print("wrote", len(data), "cases")

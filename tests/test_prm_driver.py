@@ -1,0 +1,230 @@
+"""Parameter-file driver (lsmf_run) on the host: parsing, set-up, header.
+
+  * the reference's own test parameter files (tests/golden/prm, copied from
+    the reference's tests/) parse, and --dry-run prints the blessed header
+    (reference volume, active cells, DoFs) byte for byte
+  * the set-up derived from a file (levels, coarse mesh, boundary
+    conditions, tractions at the first step) matches what the file states
+  * ParameterHandler syntax: include (read in the current subsection),
+    comments, continued lines; undeclared entries, values off their
+    pattern and unbalanced subsections are errors with the reference's
+    messages, reported with exit status 1 and the "Exception on processing"
+    block of src/main.cc
+  * seeded material fields written by the driver equal the Python workload
+    generator's (same SplitMix64 stream)
+  * cases outside this build (Holes geometry, Formulation 0, assembled
+    solvers) fail with a clear message
+No GPU is used: --dry-run stops before any device call.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import lsmf
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PRM = os.path.join(HERE, "golden", "prm")
+EXE = os.path.join(os.path.dirname(HERE), "large-strain-matrix-free_amd", "lsmf", "lsmf_run")
+
+
+def run(args, cwd=None):
+    p = subprocess.run([EXE] + args, capture_output=True, text=True, cwd=cwd, timeout=120)
+    return p.returncode, p.stdout, p.stderr
+
+
+def describe(path):
+    rc, out, err = run(["--dry-run", "--describe-setup", path])
+    assert rc == 0, err
+    js = [l for l in err.splitlines() if l.startswith("{")]
+    return out, json.loads(js[-1])
+
+
+def header(output_text):
+    """the lines before the first time step"""
+    return output_text.split("\nTimestep")[0].rstrip("\n") + "\n"
+
+
+@pytest.mark.parametrize("name", ["cook_mf_form_1", "cook_mf_gmg_form_1", "cook_mat_form_1"])
+def test_reference_prm_header_matches_blessed_output(name):
+    out, js = describe(os.path.join(PRM, name + ".prm"))
+    blessed = open(os.path.join(PRM, name + ".output")).read()
+    assert out == header(blessed)
+
+
+def test_cook_gmg_setup():
+    _, js = describe(os.path.join(PRM, "cook_mf_gmg_form_1.prm"))
+    assert js["n_levels"] == 5 and js["n_cells_coarse"] == [2, 2, 1] and js["n_cells_fine"] == [32, 32, 1]
+    assert js["preconditioner"] == 0 and js["mg_precision"] == 1
+    assert js["dirichlet_components"] == [3, 0, 0, 0]  # id 1 = -X side, both components
+    assert js["neumann_sides"] == 2  # id 11 = +X side
+    assert js["traction"][1] == [0.0, 0.1 / 16.0e-6]  # t/(16.0e-6) at t = dt
+    assert js["max_lin_iterations"] == 2178  # multiplier 1 x n_dofs
+    assert js["increment_nonzero"] == 0
+    assert js["caching"] == 0  # scalar: the reference's default "MF caching"
+
+
+def test_cook_jacobi_setup():
+    _, js = describe(os.path.join(PRM, "cook_mf_form_1.prm"))
+    assert js["n_levels"] == 1 and js["n_cells_coarse"] == [32, 32, 1]
+    assert js["preconditioner"] == 1 and js["relaxation"] == 0.65
+
+
+def write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+BOX = """
+subsection Geometry
+  set Type = Box
+  set Dimension = {dim}
+  set Elements per edge = {E}
+  set Global refinement = {R}
+  set Grid scale = 1
+end
+subsection Finite element system
+  set Polynomial degree = {p}
+  set Quadrature order = {q}
+end
+subsection Material properties
+  set Formulation = 1
+  set Poisson's ratio = {nu}
+  set Young's modulus = 1
+  {material}
+end
+subsection Linear solver
+  set Solver type = MF_CG
+  set Preconditioner type = gmg
+  set MF caching = tensor4
+end
+subsection Boundary conditions
+  set Dirichlet IDs and expressions = 0:0,0{z0};1:{inc}
+  set Dirichlet IDs and component mask = 0:true,true{zt};1:true,false{zf}
+end
+"""
+
+
+def box(dim=2, E=2, R=4, p=2, nu=0.3, material="", inc=None):
+    z0, zt, zf = ("", "", "") if dim == 2 else (",0", ",true", ",false")
+    if inc is None:
+        inc = "0.01,0" if dim == 2 else "0.01,0,0"
+    return BOX.format(dim=dim, E=E, R=R, p=p, q=p + 1, nu=nu, material=material, z0=z0, zt=zt, zf=zf, inc=inc)
+
+
+def test_box_config0_materials_equal_workload_generator(tmp_path):
+    """2D unit square 32^2 Q2 with 8 seeded inclusions (radius 0.05, 10x)"""
+    cfg = lsmf.CONFIGS[0]
+    text = box(material=f"""set Material field = inclusions
+  set Inclusion count = {cfg['n_incl']}
+  set Inclusion radius = {cfg['radius']}
+  set Inclusion factor = {cfg['factor']}
+  set Random seed = {lsmf.workload.SEED}""")
+    path = write(tmp_path, "c0.prm", text)
+    dump = str(tmp_path / "mat.bin")
+    rc, out, err = run(["--dry-run", "--describe-setup", "--dump-material", dump, path])
+    assert rc == 0, err
+    js = json.loads([l for l in err.splitlines() if l.startswith("{")][-1])
+    assert js["n_levels"] == 5 and js["n_cells_fine"] == [32, 32, 1] and js["n_dofs"] == 2 * 65 * 65
+    assert js["dirichlet_components"] == [3, 1, 0, 0]
+    assert js["increment_nonzero"] == 65 and abs(js["increment_sum"] - 0.65) < 1e-12
+    assert "Reference volume: 1\n" in out
+    m = np.fromfile(dump)
+    mesh, mu, lam = lsmf.build_config_mesh(0)
+    # the config-0 generator is seeded with SEED + 0
+    assert np.array_equal(m[:mesh.n_cells], mu) and np.array_equal(m[mesh.n_cells:], lam)
+    assert (mu > mu.min()).sum() > 0
+
+
+def test_box_lognormal_materials_equal_workload_generator(tmp_path):
+    text = box(dim=3, E=1, R=3, material="""set Material field = lognormal
+  set Log-normal sigma = 0.5
+  set Random seed = 77""")
+    path = write(tmp_path, "ln.prm", text)
+    dump = str(tmp_path / "mat.bin")
+    rc, out, err = run(["--dry-run", "--dump-material", dump, path])
+    assert rc == 0, err
+    m = np.fromfile(dump)
+    n = 8 ** 3
+    mu, lam = lsmf.lognormal_material(n, 1.0, 0.3, 0.5, 77)
+    # cos/sin/log of two libms: equal to the last bits
+    assert np.allclose(m[:n], mu, rtol=1e-14, atol=0) and np.allclose(m[n:], lam, rtol=1e-14, atol=0)
+
+
+def test_dirichlet_expression_in_space_and_time(tmp_path):
+    """u_x = 0.01 * t * (1 + y) on x = 1: the increment at t = dt is summed
+    over the side's support points"""
+    text = box(E=4, R=0, inc="0.01*t*(1+y),0") + "subsection Time\n  set Time step size = 0.25\nend\n"
+    _, js = describe(write(tmp_path, "d.prm", text))
+    y = np.linspace(0, 1, 4 * 2 + 1)
+    assert abs(js["increment_sum"] - np.sum(0.01 * 0.25 * (1 + y))) < 1e-15
+    assert js["preconditioner"] == 0 and js["n_levels"] == 1
+
+
+def test_neumann_varying_along_side_is_rejected(tmp_path):
+    text = box(E=2, R=1) + "subsection Boundary conditions\n  set Neumann IDs and expressions = 3:0,x*t\nend\n"
+    rc, out, err = run(["--dry-run", "--describe-setup", write(tmp_path, "n.prm", text)])
+    assert rc == 1 and "varies along the boundary" in err and "Exception on processing" in err
+
+
+def test_include_comments_and_continued_lines(tmp_path):
+    write(tmp_path, "inc.prm", "  set Polynomial degree = 2 # inside the caller's subsection\n"
+                               "  set Quadrature order = \\\n      3\n")
+    main = box(E=2, R=1).replace("  set Polynomial degree = 2\n  set Quadrature order = 3\n",
+                                 "  include inc.prm\n")
+    assert "include inc.prm" in main
+    rc, out, err = run(["--dry-run", "--describe-setup", write(tmp_path, "main.prm", main)], cwd=str(tmp_path))
+    assert rc == 0, err
+    js = json.loads([l for l in err.splitlines() if l.startswith("{")][-1])
+    assert js["degree"] == 2 and js["n_q_1d"] == 3
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("subsection Geometry\n  set Elements per edges = 3\nend\n", "No entry with name <Elements per edges>"),
+    ("subsection Geometry\n  set Elements per edge = -3\nend\n", "does not match the given pattern"),
+    ("subsection Material properties\n  set Poisson's ratio = 0.7\nend\n", "does not match the given pattern"),
+    ("subsection Geometry\n  set Type = Sphere\nend\n", "does not match the given pattern"),
+    ("subsection Geometry\n  set Dimension = 2\n", "unequal numbers of 'subsection' and 'end'"),
+    ("end\n", "There is no subsection to leave here"),
+    ("subsection Geometri\nend\n", "There is no subsection <Geometri> declared here"),
+    ("subsection Boundary conditions\n  set Neumann IDs and expressions = 11:0,t/(\nend\n",
+     "does not match the given pattern"),
+    ("subsection Boundary conditions\n  set Dirichlet IDs and component mask = 1:true\nend\n",
+     "does not match the given pattern"),
+    ("bogus line\n", "matched nothing known"),
+])
+def test_parameter_errors(tmp_path, text, msg):
+    rc, out, err = run(["--dry-run", write(tmp_path, "e.prm", text)])
+    assert rc == 1 and msg in err, err
+    assert "Exception on processing" in err and "Aborting!" in err
+
+
+@pytest.mark.parametrize("name,msg", [
+    ("cook_mf", "Formulation 0 is not supported"),
+    ("cook_3d_quad_mf", "Formulation 0 is not supported"),
+])
+def test_unsupported_reference_cases_fail_clearly(name, msg):
+    rc, out, err = run(["--dry-run", os.path.join(PRM, name + ".prm")])
+    assert rc == 1 and msg in err
+
+
+def test_unsupported_settings(tmp_path):
+    for extra, msg in [("subsection Geometry\n  set Type = Holes\nend\n", "Holes"),
+                       ("subsection Linear solver\n  set Solver type = Direct\nend\n", "Solver type Direct"),
+                       ("subsection Linear solver\n  set Preconditioner type = amg\nend\n", "Preconditioner type amg"),
+                       ("subsection Finite element system\n  set Quadrature order = 5\nend\n", "n_q_points_1d=5")]:
+        rc, out, err = run(["--dry-run", write(tmp_path, "u.prm", box() + extra)])
+        assert rc == 1 and msg in err, (extra, err)
+
+
+def test_print_parameters_lists_every_entry():
+    rc, out, err = run(["--print-parameters", os.path.join(PRM, "cook_mf_gmg_form_1.prm")])
+    assert rc == 0
+    lines = dict(l.split(" = ", 1) if " = " in l else (l.rstrip(" ="), "") for l in out.splitlines())
+    assert lines["Linear solver/Preconditioner type"] == "gmg"
+    assert lines["Boundary conditions/Neumann IDs and expressions"] == "11:0,t/(16.0e-6)"
+    assert lines["Nonlinear solver/Absolute tolerance force"] == "0"  # default
+    assert len(lines) >= 45
