@@ -50,6 +50,7 @@ def lib():
             "orc_op_set_material": (i, [vp, vp, vp]),
             "orc_op_set_linearization": (i, [vp, vp]),
             "orc_op_cache": (i, [vp]),
+            "orc_op_set_formulation": (i, [vp, i]),
             "orc_op_vmult": (i, [vp, vp, vp, i, i]),
             "orc_op_diagonal": (i, [vp, vp]),
             "orc_op_residual": (i, [vp, vp]),
@@ -88,9 +89,10 @@ def num_threads():
 
 
 class OracleOperator:
-    """FP64 CPU restatement of NeoHookOperator (formulation 1)."""
+    """FP64 CPU restatement of NeoHookOperator (formulation 1 or 0)."""
 
-    def __init__(self, dim, degree, cell_nodes, coords, constrained=None, caching="tensor4", n_q_1d=None):
+    def __init__(self, dim, degree, cell_nodes, coords, constrained=None, caching="tensor4", n_q_1d=None,
+                 formulation=1):
         self.dim, self.degree = dim, degree
         self.nq = degree + 1 if n_q_1d is None else n_q_1d
         self.cell_nodes = np.ascontiguousarray(cell_nodes, dtype=np.int32)
@@ -105,6 +107,8 @@ class OracleOperator:
         if not self._h:
             raise ValueError(f"oracle: unsupported (dim, degree, nq) = ({dim}, {degree}, {self.nq})")
         self._destroy = lib().orc_op_destroy
+        if lib().orc_op_set_formulation(self._h, int(formulation)) != 0:
+            raise ValueError(f"oracle: unknown formulation {formulation}")
 
     def __del__(self):
         if getattr(self, "_h", None):

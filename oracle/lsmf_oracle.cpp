@@ -7,11 +7,13 @@
 // it, and it is never the thing measured as the GPU result.
 //
 // What it restates (reference paths relative to the reference repository):
-//   * material law, formulation 1 ............ include/material.h:78-163, 171-250
+//   * material law, formulations 1 and 0 ..... include/material.h:78-163, 171-250,
+//                                              260-295 (Psi_vol and derivatives)
 //   * NeoHookOperator::cache() ............... include/mf_nh_operator.h:447-592
 //   * NeoHookOperator::vmult_add() ........... include/mf_nh_operator.h:646-706
 //   * local_apply_cell / do_operation_on_cell  include/mf_nh_operator.h:710-785,
-//                                              922-1499 (formulation 1 branches)
+//                                              922-1499 (formulation 1 branches,
+//                                              formulation 0 at :1001-1181)
 //   * compute_diagonal / local_diagonal_cell . include/mf_nh_operator.h:789-918,
 //                                              1503-1557
 //   * residual in assemble_system() .......... include/mf_elasticity.h:2758-2804,
@@ -189,6 +191,10 @@ struct OpBase {
   virtual int n_colours() = 0;
   virtual int min_detF(double *out) = 0;
   int dim = 0, p = 0, nq = 0, n_cells = 0, n_nodes = 0, caching = 0;
+  // Material_Compressible_Neo_Hook_One_Field's formulation (include/material.h:82-94):
+  // 1 = mu/2 (tr C - d - 2 ln J) + lambda ln^2 J; 0 = volumetric/isochoric split
+  // with Psi_vol = kappa/4 (J^2 - 1 - 2 ln J), kappa = lambda + 2 mu / 3
+  int formulation = 1;
   std::vector<unsigned char> cons; // per DoF
 };
 
@@ -585,6 +591,12 @@ template <int DIM, int P, int NQ> struct Op : OpBase {
         const double *F = &Fq[q * DD];
         const double J = det<DIM>(F);
         if (!(J > 0)) { ++bad; continue; }
+        if (formulation == 0) {
+          // formulation 0 caches J^{-1/dim} whatever the caching
+          // (include/mf_nh_operator.h:467-487)
+          c_s[cq] = std::pow(J, -1.0 / DIM);
+          continue;
+        }
         const double scalar = m - 2.0 * l * std::log(J);
         double tau[DD];
         matmul_bt<DIM>(F, F, tau);  // b = F F^T
@@ -633,7 +645,67 @@ template <int DIM, int P, int NQ> struct Op : OpBase {
   }
 
   // do_operation_on_cell for formulation 1 (include/mf_nh_operator.h:1182-1455)
+  // Kirchhoff stress and the action of J c on sym(g) for formulation 0
+  // (include/material.h:116-141, 171-226; include/mf_nh_operator.h:1005-1180):
+  // tau = (J dPsi/dJ - tr(tau_bar)/d) I + tau_bar, tau_bar = mu b_bar,
+  // b_bar = J^{-2/d} b.  s_bar = J^{-1/d}.
+  static void form0_tau(const double *F, double s_bar, double m, double l, double *tau, double *tau_iso,
+                        double &dpsi_J, double &kappa, double &tr_tau_bar) {
+    const double J = det<DIM>(F);
+    kappa = l + 2.0 * m / 3.0;
+    double b[DD];
+    matmul_bt<DIM>(F, F, b);
+    tr_tau_bar = 0;
+    for (int k = 0; k < DD; ++k) tau[k] = m * s_bar * s_bar * b[k];  // tau_bar
+    for (int d = 0; d < DIM; ++d) tr_tau_bar += tau[d * DIM + d];
+    for (int k = 0; k < DD; ++k) tau_iso[k] = tau[k];
+    for (int d = 0; d < DIM; ++d) tau_iso[d * DIM + d] -= tr_tau_bar / DIM;
+    dpsi_J = 0.5 * kappa * (J - 1.0 / J) * J;
+    for (int d = 0; d < DIM; ++d) tau[d * DIM + d] += dpsi_J - tr_tau_bar / DIM;
+  }
+
+  void cell_op_form0(int cell, const double *in, double *out) const {
+    double G[NQP * DD], Phi[NQP * DD], Gu[NQP * DD], ul[DIM * NDOF1];
+    grad(in, G);
+    gather(u.data(), cell, ul, false);
+    grad(ul, Gu);
+    const double m = mu[cell], l = lam[cell];
+    for (int q = 0; q < NQP; ++q) {
+      const size_t cq = (size_t)cell * NQP + q;
+      double F[DD], g[DD], s[DD], tau[DD], tau_iso[DD], jc[DD], t1[DD], Q[DD];
+      matmul<DIM>(&Gu[q * DD], &jinv_ref[cq * DD], F);
+      for (int d = 0; d < DIM; ++d) F[d * DIM + d] += 1.0;
+      const double J = det<DIM>(F);
+      double dpsi_J, kappa, tr_tau_bar;
+      form0_tau(F, c_s[cq], m, l, tau, tau_iso, dpsi_J, kappa, tr_tau_bar);
+      matmul<DIM>(&G[q * DD], &jinv_cur[cq * DD], g);
+      for (int i = 0; i < DIM; ++i)
+        for (int j = 0; j < DIM; ++j) s[i * DIM + j] = 0.5 * (g[i * DIM + j] + g[j * DIM + i]);
+      double tr = 0, tiso_s = 0;
+      for (int d = 0; d < DIM; ++d) tr += s[d * DIM + d];
+      for (int k = 0; k < DD; ++k) tiso_s += tau_iso[k] * s[k];
+      const double d2psi = 0.5 * kappa * (1.0 + 1.0 / (J * J));
+      const double dpsi = 0.5 * kappa * (J - 1.0 / J);
+      for (int i = 0; i < DIM; ++i)
+        for (int j = 0; j < DIM; ++j) {
+          const double dev = s[i * DIM + j] - (i == j ? tr / DIM : 0.0);
+          jc[i * DIM + j] = -2.0 * dpsi_J * s[i * DIM + j] + (i == j ? J * (dpsi + J * d2psi) * tr : 0.0) +
+                            (2.0 / DIM) * tr_tau_bar * dev - (2.0 / DIM) * tr * tau_iso[i * DIM + j] -
+                            (i == j ? (2.0 / DIM) * tiso_s : 0.0);
+        }
+      matmul<DIM>(g, tau, t1);  // geometric part grad v tau
+      for (int k = 0; k < DD; ++k) Q[k] = (jc[k] + t1[k]) / J;
+      matmul_bt<DIM>(Q, &jinv_cur[cq * DD], t1);
+      for (int k = 0; k < DD; ++k) Phi[q * DD + k] = t1[k] * jxw_cur[cq];
+    }
+    grad_T(Phi, out);
+  }
+
   void cell_op(int cell, const double *in, double *out) const {
+    if (formulation == 0) {
+      cell_op_form0(cell, in, out);
+      return;
+    }
     double G[NQP * DD], Phi[NQP * DD];
     grad(in, G);
     double Gu[NQP * DD];
@@ -801,10 +873,15 @@ template <int DIM, int P, int NQ> struct Op : OpBase {
           double Fi[DD], tau[DD], P1[DD], t1[DD];
           const double J = det<DIM>(F);
           inv<DIM>(F, Fi);
-          matmul_bt<DIM>(F, F, tau);
-          for (int k2 = 0; k2 < DD; ++k2) tau[k2] *= m;
-          const double scalar = m - 2.0 * l * std::log(J);
-          for (int d = 0; d < DIM; ++d) tau[d * DIM + d] -= scalar;
+          if (formulation == 0) {
+            double tau_iso[DD], dpsi_J, kappa, tr_tau_bar;
+            form0_tau(F, std::pow(J, -1.0 / DIM), m, l, tau, tau_iso, dpsi_J, kappa, tr_tau_bar);
+          } else {
+            matmul_bt<DIM>(F, F, tau);
+            for (int k2 = 0; k2 < DD; ++k2) tau[k2] *= m;
+            const double scalar = m - 2.0 * l * std::log(J);
+            for (int d = 0; d < DIM; ++d) tau[d * DIM + d] -= scalar;
+          }
           matmul_bt<DIM>(tau, Fi, P1);                       // P = tau F^{-T}
           matmul_bt<DIM>(P1, &jinv_ref[cq * DD], t1);        // to unit-cell gradients
           for (int k2 = 0; k2 < DD; ++k2) Phi[q * DD + k2] = -t1[k2] * jxw_ref[cq];
@@ -1001,6 +1078,11 @@ int orc_op_set_material(void *op, const double *mu, const double *lambda) {
 }
 int orc_op_set_linearization(void *op, const double *u) { return static_cast<orc::OpBase *>(op)->set_linearization(u); }
 int orc_op_cache(void *op) { return static_cast<orc::OpBase *>(op)->cache(); }
+int orc_op_set_formulation(void *op, int formulation) {
+  if (formulation != 0 && formulation != 1) return -1;
+  static_cast<orc::OpBase *>(op)->formulation = formulation;
+  return 0;
+}
 int orc_op_vmult(void *op, double *dst, const double *src, int add, int plain) {
   return static_cast<orc::OpBase *>(op)->vmult(dst, src, add, plain);
 }

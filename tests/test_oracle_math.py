@@ -10,6 +10,9 @@
     field) vanishes at interior DoFs
   * central finite differences of the residual equal -A v
   * diagonal = e_i.A e_i; all caching variants agree on a heterogeneous mesh
+  * formulation 0 (volumetric/isochoric split): at u = 0 linear elasticity
+    with (mu, kappa - 2 mu / d), kappa = lambda + 2 mu / 3; patch test; the
+    residual's finite differences equal -A v
 """
 import numpy as np
 import pytest
@@ -29,8 +32,9 @@ def hetero_problem(dim=3, p=2, n=3, seed=5):
     return mesh, mu, lam, u, cons
 
 
-def make_op(mesh, mu, lam, u, cons, caching):
-    op = oracle.OracleOperator(mesh.dim, mesh.degree, mesh.cell_nodes, mesh.coords, cons, caching=caching)
+def make_op(mesh, mu, lam, u, cons, caching, formulation=1):
+    op = oracle.OracleOperator(mesh.dim, mesh.degree, mesh.cell_nodes, mesh.coords, cons, caching=caching,
+                               formulation=formulation)
     op.set_material(mu, lam)
     op.set_linearization(u)
     op.cache()
@@ -155,12 +159,13 @@ def test_patch_test_homogeneous_deformation():
     assert abs(op.min_det_F() - np.linalg.det(F0)) < 1e-13
 
 
+@pytest.mark.parametrize("formulation", [1, 0])
 @pytest.mark.parametrize("dim,p", [(2, 2), (3, 2)])
-def test_residual_finite_differences_equal_tangent(dim, p):
+def test_residual_finite_differences_equal_tangent(dim, p, formulation):
     mesh, mu, lam, u, cons = hetero_problem(dim, p, 2)
     v = np.random.default_rng(4).standard_normal(mesh.n_dofs)
     h = 1e-6
-    op = make_op(mesh, mu, lam, u, cons, "tensor4")
+    op = make_op(mesh, mu, lam, u, cons, "tensor4", formulation)
     Av = op.vmult(v, plain=True)
     op.set_linearization(u + h * v)
     rp = op.residual()
@@ -168,3 +173,39 @@ def test_residual_finite_differences_equal_tangent(dim, p):
     rm = op.residual()
     fd = (rp - rm) / (2 * h)
     assert np.allclose(fd, -Av, rtol=1e-6, atol=1e-7 * np.abs(Av).max())
+
+
+@pytest.mark.parametrize("dim,p", [(2, 2), (3, 2), (3, 1)])
+def test_formulation0_at_zero_displacement_is_linear_elasticity(dim, p):
+    """tau_bar = mu I, tau = 0 and J c = 2 mu dev + kappa tr I at F = I"""
+    cfe = fe.CellFE(dim, p, p + 1, fe.unit_cell_vertices(dim) * 0.7)
+    X = cfe.support_points()
+    mu, lam = 1.3, 2.1
+    kappa = lam + 2 * mu / 3
+    K = voigt_linear_elasticity(cfe, mu, kappa - 2 * mu / dim)
+    v = np.random.default_rng(7).random(len(X) * dim)
+    op = oracle.OracleOperator(dim, p, np.arange(len(X), dtype=np.int32)[None], X, None, caching="scalar",
+                               formulation=0)
+    op.set_material([mu], [lam])
+    op.set_linearization(np.zeros(len(X) * dim))
+    op.cache()
+    assert np.allclose(op.vmult(v), K @ v, rtol=1e-12, atol=1e-12 * np.abs(K @ v).max())
+    assert np.abs(op.residual()).max() < 1e-13
+
+
+def test_formulation0_patch_test_and_symmetry():
+    mesh = lsmf.Mesh(3, 2, [3, 3, 3])
+    F0 = np.array([[1.05, 0.02, 0.0], [0.01, 0.97, 0.03], [0.0, -0.02, 1.01]])
+    u = (mesh.coords @ (F0 - np.eye(3)).T).reshape(-1)
+    mu, lam = np.full(mesh.n_cells, 0.7), np.full(mesh.n_cells, 1.1)
+    B = np.array([[0.1, 0.2, -0.1], [0.0, 0.3, 0.05], [0.2, -0.1, 0.1]])
+    v = (mesh.coords @ B.T).reshape(-1)
+    interior = np.all((mesh.coords > 1e-12) & (mesh.coords < 1 - 1e-12), axis=1)
+    idof = (3 * np.where(interior)[0][:, None] + np.arange(3)).reshape(-1)
+    op = make_op(mesh, mu, lam, u, None, "scalar", formulation=0)
+    Av = op.vmult(v, plain=True)
+    assert np.abs(Av[idof]).max() < 1e-12 * np.abs(Av).max()
+    r = op.residual()
+    assert np.abs(r[idof]).max() < 1e-12 * np.abs(r).max()
+    w = np.random.default_rng(2).random(mesh.n_dofs)
+    assert abs(w @ Av - v @ op.vmult(w, plain=True)) < 1e-12 * abs(w @ Av)
