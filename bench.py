@@ -112,6 +112,84 @@ def newton_solve(cfg_id, n_override, mu, lam, precision, tol, coarse="direct", v
     return {"create_s": t_create, "stats": stats, "u": u, "n_levels": n_levels}
 
 
+def time_ms(fn, reps, torch):
+    """average ms of fn() over reps launches on the current stream (HIP events)"""
+    stream = torch.cuda.current_stream()
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def variant_sweep(cfg_id, reps):
+    """vmult throughput of every caching variant (and formulation 0) on one
+    configuration, FP64 and FP32"""
+    import torch
+    import lsmf
+    mesh, mu, lam = lsmf.build_config_mesh(cfg_id)
+    u, _ = lsmf.smooth_displacement(mesh.coords, 0.02, lsmf.workload.SEED + cfg_id)
+    cons = mesh.constrained_mask({0: list(range(mesh.dim)), 1: [0]})
+    dev = torch.device("cuda:0")
+    ud = torch.tensor(u, device=dev)
+    out = {"workload": f"config {cfg_id}: {mesh.n_cells_dir[0]}^{mesh.dim} Q{mesh.degree}, {mesh.n_dofs} DoF",
+           "gdofs": {}, "cache_bytes": {}}
+    cases = [(c, p, 1) for c in ("tensor4", "tensor2", "scalar", "tensor4_ns", "scalar_referential")
+             for p in ("fp64", "fp32")] + [("scalar", "fp64", 0), ("tensor4", "fp64", 0)]
+    for caching, prec, form in cases:
+        op = lsmf.NeoHookOperator(mesh, precision=prec, caching=caching, constrained=cons, formulation=form)
+        op.set_material(mu, lam)
+        op.set_linearization(ud)
+        op.cache()
+        dt = torch.float64 if prec == "fp64" else torch.float32
+        x = torch.rand(mesh.n_dofs, dtype=dt, device=dev)
+        y = torch.empty_like(x)
+        ms = time_ms(lambda: op.vmult(y, x), reps, torch)
+        key = f"{caching}_{prec}" + ("_formulation0" if form == 0 else "")
+        out["gdofs"][key] = round(mesh.n_dofs / ms / 1e6, 3)
+        out["cache_bytes"][key] = op.memory_consumption()
+        del op, x, y
+    torch.cuda.empty_cache()
+    return out
+
+
+def adjoint_trade(cfg_id, reps):
+    """residual VJP that recomputes the kinematics (no stored state) against
+    the tensor4 cache a cached backward would keep: time and bytes"""
+    import torch
+    import lsmf
+    mesh, mu, lam = lsmf.build_config_mesh(cfg_id)
+    u, _ = lsmf.smooth_displacement(mesh.coords, 0.01, lsmf.workload.SEED + cfg_id)
+    dev = torch.device("cuda:0")
+    ud = torch.tensor(u, device=dev)
+    op = lsmf.NeoHookOperator(mesh, precision="fp64", caching="tensor4")
+    op.set_material(mu, lam)
+    op.set_linearization(ud)
+    w = torch.rand(mesh.n_dofs, dtype=torch.float64, device=dev)
+    du, r = torch.empty_like(w), torch.empty_like(w)
+    dmu = torch.empty(mesh.n_cells, dtype=torch.float64, device=dev)
+    dla = torch.empty_like(dmu)
+    t_res = time_ms(lambda: op.residual(r), reps, torch)
+    t_vjp = time_ms(lambda: op.residual_vjp(w, du, dmu, dla), reps, torch)
+    t_cache = time_ms(lambda: op.cache(), max(1, reps // 4), torch)
+    t_vmult = time_ms(lambda: op.vmult(r, w, mask="plain"), reps, torch)
+    n_qp = mesh.n_cells * (mesh.degree + 1) ** mesh.dim
+    out = {"workload": f"config {cfg_id}: {mesh.n_cells_dir[0]}^{mesh.dim} Q{mesh.degree} log-normal mu/lambda, "
+                       f"{mesh.n_dofs} DoF, FP64",
+           "residual_ms": round(t_res, 4), "vjp_recompute_ms": round(t_vjp, 4),
+           "tensor4_cache_build_ms": round(t_cache, 4), "tensor4_vmult_ms": round(t_vmult, 4),
+           "vjp_state_bytes": 0, "tensor4_cache_bytes": 36 * 8 * n_qp,
+           "note": "VJP w.r.t. u, mu, lambda recomputes F, J, ln J per point; the cached alternative "
+                   "(tensor4 cache + per-point b - I and ln J) would hold 36 + 7 FP64 values per point"}
+    del op
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +206,8 @@ def main():
     ap.add_argument("--solve-tol", type=float, default=1e-8)
     ap.add_argument("--coarse", default="direct", choices=["direct", "chebyshev"],
                     help="coarse-level solver of the V-cycle (chebyshev = the reference's choice)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the caching-variant sweep (32^3 Q2) and the adjoint time/memory trade (48^3 Q2)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, run 3 plain device copies of the source vector (known bytes) so "
                          "HBM counters (FETCH_SIZE / WRITE_SIZE) can be scaled")
@@ -219,7 +299,8 @@ def main():
         solve = {
             "fp32_gmg_s": round(t32, 3), "fp64_gmg_s": round(t64, 3), "speedup_fp32_vs_fp64": round(t64 / t32, 3),
             "tol": args.solve_tol, "load_steps": len(res["fp32"]["stats"]), "mg_levels": res["fp32"]["n_levels"],
-            "coarse_solver": args.coarse, "smoother": "Chebyshev degree 5, range 20, 30 eigen-CG iterations",
+            "coarse_solver": args.coarse,
+            "smoother": f"Chebyshev degree {cfg.get('cheb_degree', 5)}, range 20, 30 eigen-CG iterations",
             "newton_iterations": {p: [s["newton_iterations"] for s in res[p]["stats"]] for p in res},
             "cg_iterations": {p: [s["linear_iterations"] for s in res[p]["stats"]] for p in res},
             "final_rel_residual": {p: [s["residual_final"] / s["residual_0"] for s in res[p]["stats"]] for p in res},
@@ -229,6 +310,15 @@ def main():
             "timed": "Newton loop of all load steps (residual, caches, level setup, CG); hierarchy creation excluded",
         }
         del res
+
+    variants = adjoint = None
+    if not args.no_extras and args.n is None:
+        progress("caching-variant sweep on config 1 (32^3 Q2)")
+        variants = variant_sweep(1, 20)
+        progress(f"  GDoF/s {variants['gdofs']}")
+        progress("adjoint time/memory trade on config 3 (48^3 Q2)")
+        adjoint = adjoint_trade(3, 20)
+        progress(f"  {adjoint}")
 
     cpu = None
     if not args.no_cpu:
@@ -260,6 +350,8 @@ def main():
                      "algorithmic_bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu,
         "newton_solve": solve,
+        "caching_variants": variants,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
 

@@ -103,7 +103,16 @@ typedef struct {
   int n_q_1d;    /* QGauss points per direction (n_q_points_1d) */
   int precision; /* LSMF_FP64 or LSMF_FP32 */
   int caching;   /* LSMF_CACHING_* */
+  int layout;    /* LSMF_LAYOUT_AUTO (0) or LSMF_LAYOUT_CELLS */
 } lsmf_op_options;
+
+/* Cell loop layout.  AUTO: on a structured mesh (the lexicographic numbering
+ * lsmf_mesh_create produces, detected from cell_nodes) cells are processed in
+ * tiles (3x3x1 Q2 cells in 3D, ...) whose shared nodes are gathered once and
+ * accumulated in LDS, with the tiles coloured by parity; otherwise, and with
+ * CELLS, one cell per lane group with the cells coloured (given colouring or
+ * greedy).  Both give the same operator to round-off. */
+enum { LSMF_LAYOUT_AUTO = 0, LSMF_LAYOUT_CELLS = 1 };
 
 typedef struct lsmf_op lsmf_op;
 
@@ -122,6 +131,16 @@ lsmf_status lsmf_op_destroy(lsmf_op *op);
  * case.  lambda is Lame's first parameter, lambda = 2 mu nu / (1 - 2 nu)
  * (include/material.h:82-94). */
 lsmf_status lsmf_set_material_per_cell(lsmf_op *op, const double *mu, const double *lambda);
+
+/* The formulation argument of Material_Compressible_Neo_Hook_One_Field
+ * (include/material.h:82-94), which the reference's set_material passes in:
+ * 1 (default): psi = mu/2 (tr C - d - 2 ln J) + lambda ln^2 J;
+ * 0: the volumetric/isochoric split, psi = kappa/4 (J^2 - 1 - 2 ln J)
+ * + mu/2 (tr b_bar - d), kappa = lambda + 2 mu / 3, whose operator caches
+ * J^{-1/d} only (include/mf_nh_operator.h:467-487, 1001-1181).  Formulation 0
+ * runs with scalar or tensor4 caching (LSMF_ERR_UNSUPPORTED otherwise) and
+ * has no residual VJP.  Call before cache(). */
+lsmf_status lsmf_set_formulation(lsmf_op *op, int formulation);
 
 /* The displacement argument of NeoHookOperator::initialize
  * (include/mf_nh_operator.h:397, 404): the linearisation state.  u_dev is a
@@ -264,6 +283,7 @@ typedef struct {
   int preconditioner;            /* LSMF_PRECONDITIONER_GMG / _JACOBI / _NONE ("Preconditioner type") */
   double preconditioner_relaxation; /* Jacobi damping omega ("Preconditioner relaxation") */
   double tol_f_abs, tol_u_abs;   /* "Absolute tolerance force" / "... displacement" (:2262-2266) */
+  int formulation;               /* material formulation, "Formulation" (1 or 0) */
 } lsmf_solver_options;
 
 enum { LSMF_PRECONDITIONER_GMG = 0, LSMF_PRECONDITIONER_JACOBI = 1, LSMF_PRECONDITIONER_NONE = 2 };

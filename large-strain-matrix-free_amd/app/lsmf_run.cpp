@@ -117,9 +117,6 @@ void setup(Problem &pr, int dim, int degree, int n_q) {
   pr.n_q = n_q;
   if (P.type == "Holes")
     throw std::runtime_error("Geometry type Holes (curved manifold meshes) is not supported by this build");
-  if (P.material_formulation != 1)
-    throw std::runtime_error("Formulation " + std::to_string(P.material_formulation) +
-                             " is not supported: this build implements Formulation = 1");
   if (P.type_lin == "CG")
     std::cerr << "Note: Solver type CG (assembled matrix) runs on the matrix-free operator here" << std::endl;
   else if (P.type_lin != "MF_CG")
@@ -133,6 +130,14 @@ void setup(Problem &pr, int dim, int degree, int n_q) {
   else if (P.mf_caching == "tensor4") caching = LSMF_CACHING_TENSOR4;
   else if (P.mf_caching == "tensor4_ns") caching = LSMF_CACHING_TENSOR4_NS;
   else caching = LSMF_CACHING_SCALAR_REFERENTIAL;
+  // formulation 0 ignores the caching choice in the reference (its operator
+  // caches J^{-1/d} only, include/mf_nh_operator.h:467-487); here it runs
+  // with the scalar cache, or with the tensor4 cache when that is asked for
+  if (P.material_formulation == 0 && caching != LSMF_CACHING_SCALAR && caching != LSMF_CACHING_TENSOR4) {
+    std::cerr << "Note: Formulation 0 runs with scalar caching (MF caching = " << P.mf_caching << " asked)"
+              << std::endl;
+    caching = LSMF_CACHING_SCALAR;
+  }
   if (!P.mf_coarse_chebyshev)
     throw std::runtime_error("MF Chebyshev coarse = false (CG coarse solver) is not supported");
 
@@ -228,6 +233,7 @@ void setup(Problem &pr, int dim, int degree, int n_q) {
   o.preconditioner = gmg ? LSMF_PRECONDITIONER_GMG
                          : (P.preconditioner_type == "jacobi" ? LSMF_PRECONDITIONER_JACOBI : LSMF_PRECONDITIONER_NONE);
   o.preconditioner_relaxation = P.preconditioner_relaxation;
+  o.formulation = P.material_formulation;
   for (const auto &kv : P.dirichlet) {
     auto m = P.dirichlet_mask.find(kv.first);
     if (m == P.dirichlet_mask.end())
@@ -517,7 +523,8 @@ void describe_setup(const Problem &pr) {
      << ", \"geometry\": " << o.geometry << ", \"n_levels\": " << o.n_levels << ", \"n_cells_coarse\": ["
      << o.n_cells_coarse[0] << ", " << o.n_cells_coarse[1] << ", " << o.n_cells_coarse[2] << "], \"n_cells_fine\": ["
      << pr.n_fine[0] << ", " << pr.n_fine[1] << ", " << pr.n_fine[2] << "], \"caching\": " << o.caching
-     << ", \"mg_precision\": " << o.mg_precision << ", \"preconditioner\": " << o.preconditioner
+     << ", \"mg_precision\": " << o.mg_precision << ", \"formulation\": " << o.formulation
+     << ", \"preconditioner\": " << o.preconditioner
      << ", \"relaxation\": " << o.preconditioner_relaxation << ", \"tol_lin\": " << o.tol_lin
      << ", \"max_lin_iterations\": " << o.max_lin_iterations << ", \"max_newton\": " << o.max_newton
      << ", \"tol_f\": " << o.tol_f << ", \"tol_u\": " << o.tol_u << ", \"dt\": " << o.dt << ", \"t_end\": " << o.t_end

@@ -77,13 +77,29 @@ template <int NV> __host__ __device__ constexpr int sym_index(int v, int w) {
   return v <= w ? v * NV - v * (v - 1) / 2 + (w - v) : w * NV - w * (w - 1) / 2 + (v - w);
 }
 
+// fields per quadrature point of each cache:
+//   tensor4     K = J_cur^-1 (D2), tau (NV), C packed (NV(NV+1)/2)
+//   tensor2     K, tau, 2c, 2 lambda
+//   tensor4_ns  J_ref^-1 (D2), dP/dF packed by its major symmetry (D2(D2+1)/2)
+//   scalar      c
+//   scalar_ref  c, u at the point (DIM)
 template <int DIM, int VAR> struct CacheFields {
   static constexpr int NV = Voigt<DIM>::NV;
-  static constexpr int value = VAR == V_TENSOR4   ? DIM * DIM + NV + NV * (NV + 1) / 2
-                               : VAR == V_TENSOR2 ? DIM * DIM + NV + 2
-                               : VAR == V_SCALAR  ? 1
-                                                  : 0;
+  static constexpr int D2 = DIM * DIM;
+  static constexpr int value = VAR == V_TENSOR4      ? D2 + NV + NV * (NV + 1) / 2
+                               : VAR == V_TENSOR2    ? D2 + NV + 2
+                               : VAR == V_TENSOR4_NS ? D2 + D2 * (D2 + 1) / 2
+                               : VAR == V_SCALAR_REF ? 1 + DIM
+                               : VAR == V_SCALAR     ? 1
+                                                     : 0;
 };
+
+// 1D tables: S, D, St, Dt (N*N each), Gauss points and weights (N each),
+// then the collocation derivative on the Gauss points (N*N, used by
+// scalar_referential only)
+template <int N> __host__ __device__ constexpr int tab_size(int var) {
+  return 4 * N * N + 2 * N + (var == V_SCALAR_REF ? N * N : 0);
+}
 
 template <int DIM, int N> struct CellShape {
   static constexpr int NQP = ipow(N, DIM);
@@ -91,6 +107,22 @@ template <int DIM, int N> struct CellShape {
   static constexpr int NB = DIM * DIM;   // arrays per LDS ping-pong buffer
   static constexpr int NC = 1 << DIM;    // Q1 corners
 };
+
+// Tile of the tiled layout: TX x TY x TZ cells of a structured mesh, whose
+// (TX p + 1)(TY p + 1)(TZ p + 1) nodes are gathered once into LDS and whose
+// contributions are summed there before one write per node.  TX TY TZ = CPW.
+template <int DIM, int N> struct TileShape {
+  static constexpr int CPW = CellShape<DIM, N>::CPW;
+  static constexpr int TX = DIM == 3 ? (N == 2 ? 4 : N == 3 ? 3 : 2) : (N == 2 ? 8 : N == 3 ? 7 : 5);
+  static constexpr int TY = DIM == 3 ? (N == 2 ? 4 : N == 3 ? 3 : 1) : (N == 2 ? 8 : N == 3 ? 4 : 2);
+  static constexpr int TZ = DIM == 3 ? (N == 2 ? 2 : 1) : 1;
+  static constexpr int P = N - 1;
+  static constexpr int NX = TX * P + 1, NY = TY * P + 1, NZ = DIM == 3 ? TZ * P + 1 : 1;
+  static constexpr int TN = NX * NY * NZ;
+  static_assert(TX * TY * TZ == CPW, "tile must hold the workgroup's cells");
+  static_assert(TN <= kWG, "one lane per tile node");
+};
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;  // tile node outside the mesh
 
 // LDS-only workgroup barrier: waits for this wave's LDS traffic, not for its
 // outstanding global loads, so the per-qp cache loads issued at kernel entry
@@ -404,6 +436,136 @@ __device__ __forceinline__ void qp_tensor2(const T (&G)[DIM][DIM], const T *cch,
   flux_times_kt<DIM>(Q, K, Phi);
 }
 
+// tensor4_ns: P' = A : Grad v with Grad v = G J_ref^-1, flux P' J_ref^-T
+// (include/mf_nh_operator.h:1441-1455); A = dP/dF det(J_ref) w is packed by
+// its major symmetry over the index pairs p = DIM i + J.
+template <int DIM, typename T>
+__device__ __forceinline__ void qp_tensor4_ns(const T (&G)[DIM][DIM], const T *cch, T (&Phi)[DIM][DIM]) {
+  constexpr int D2 = DIM * DIM;
+  T Kr[DIM][DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) Kr[d][j] = cch[d * DIM + j];
+  const T *Am = cch + D2;
+  T H[DIM][DIM];
+  grad_times<DIM>(G, Kr, H);
+  T P[DIM][DIM];
+#pragma unroll
+  for (int p = 0; p < D2; ++p) {
+    T s = Am[sym_index<D2>(p, 0)] * H[0][0];
+#pragma unroll
+    for (int q = 1; q < D2; ++q) s = fma(Am[sym_index<D2>(p, q)], H[q / DIM][q % DIM], s);
+    P[p / DIM][p % DIM] = s;
+  }
+  flux_times_kt<DIM>(P, Kr, Phi);
+}
+
+// Formulation 0, the volumetric/isochoric split (include/material.h:116-141,
+// 171-226, 260-295; include/mf_nh_operator.h:1005-1180), with
+// kappa = lambda + 2 mu / 3, b_bar = sbar^2 F F^T (sbar = J^{-1/d}),
+// tau_bar = mu b_bar, tau_iso = dev(tau_bar), J dPsi_vol/dJ = kappa/2 (J^2 - 1):
+//   tau   = tau_bar + (J dPsi/dJ - tr(tau_bar)/d) I
+//   J c:s = a s + b tr(s) I - (2/d) (tr(s) tau_iso + (tau_iso : s) I)
+//   a = -2 J dPsi/dJ + (2/d) tr(tau_bar),  b = kappa J^2 - (2/d^2) tr(tau_bar)
+template <int DIM, typename T> struct Form0 {
+  T tau[DIM][DIM], tiso[DIM][DIM];
+  T a, b;
+  __device__ __forceinline__ Form0(const T (&F)[DIM][DIM], T J, T sbar, T m, T l) {
+    const T kappa = l + T(2) * m / T(3);
+    T trtb = T(0);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        T s = F[i][0] * F[j][0];
+#pragma unroll
+        for (int k = 1; k < DIM; ++k) s = fma(F[i][k], F[j][k], s);
+        tau[i][j] = m * sbar * sbar * s;
+      }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) trtb += tau[d][d];
+    const T dpsiJ = T(0.5) * kappa * (J - T(1) / J) * J;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) tiso[i][j] = tau[i][j] - (i == j ? trtb / T(DIM) : T(0));
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) tau[d][d] += dpsiJ - trtb / T(DIM);
+    a = T(-2) * dpsiJ + T(2) / T(DIM) * trtb;
+    b = kappa * J * J - T(2) / T(DIM * DIM) * trtb;
+  }
+  // Q = J c : sym(g) + g tau
+  __device__ __forceinline__ void action(const T (&g)[DIM][DIM], T (&Q)[DIM][DIM]) const {
+    T tr = g[0][0];
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) tr += g[d][d];
+    T ts = T(0);  // tau_iso : sym(g)
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) ts = fma(tiso[i][j], g[i][j], ts);
+    const T f = T(2) / T(DIM);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        T s = T(0.5) * a * (g[i][j] + g[j][i]) - f * tr * tiso[i][j];
+        if (i == j) s += b * tr - f * ts;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) s = fma(g[i][k], tau[k][j], s);
+        Q[i][j] = s;
+      }
+  }
+};
+
+// tensor4 fields of one quadrature point (the reference's tensor4 cache,
+// include/mf_nh_operator.h:548-562, and its formulation-0 counterpart):
+// T = tau det(J_ref) w (Voigt), M = C det(J_ref) w packed, C the spatial
+// tangent J c acting on sym(g); formulation 1: C = 2 c S + 2 lambda I x I.
+template <int DIM, typename T>
+__device__ __forceinline__ void tensor4_fields(const T (&F)[DIM][DIM], T J, T m, T l, T dw, int formulation,
+                                               T *Tv, T *M) {
+  constexpr int NV = Voigt<DIM>::NV;
+  if (formulation == 0) {
+    const Form0<DIM, T> f0(F, J, T(pow(double(J), -1.0 / DIM)), m, l);
+    const T f = T(2) / T(DIM);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Tv[v] = f0.tau[voigt_first<DIM>(v)][voigt_second<DIM>(v)] * dw;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int w2 = v; w2 < NV; ++w2) {
+        const int i = voigt_first<DIM>(v), j = voigt_second<DIM>(v);
+        const int k = voigt_first<DIM>(w2), ll = voigt_second<DIM>(w2);
+        const T Sijkl = T(0.5) * (T(i == k && j == ll) + T(i == ll && j == k));
+        const T dij = T(i == j), dkl = T(k == ll);
+        M[sym_index<NV>(v, w2)] =
+            (f0.a * Sijkl + f0.b * dij * dkl - f * (f0.tiso[i][j] * dkl + dij * f0.tiso[k][ll])) * dw;
+      }
+    return;
+  }
+  const T cs = m - T(2) * l * T(log(double(J)));
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const int i = voigt_first<DIM>(v), j = voigt_second<DIM>(v);
+    T s = T(0);
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) s += F[i][k] * F[j][k];
+    Tv[v] = (m * s - (i == j ? cs : T(0))) * dw;
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int w2 = v; w2 < NV; ++w2) {
+      const int i = voigt_first<DIM>(v), j = voigt_second<DIM>(v);
+      const int k = voigt_first<DIM>(w2), ll = voigt_second<DIM>(w2);
+      const T Sijkl = T(0.5) * (T(i == k && j == ll) + T(i == ll && j == k));
+      const T IxI = (i == j && k == ll) ? T(1) : T(0);
+      M[sym_index<NV>(v, w2)] = (T(2) * cs * Sijkl + T(2) * l * IxI) * dw;
+    }
+}
+
 // Q1 reference Jacobian at Gauss point (i0,i1,i2) from the cell's corner
 // coordinates X[v][i] (LDS); returns J_ref[i][d] = dX_i / dxi_d.
 template <int DIM, typename T, typename TC>
@@ -452,7 +614,10 @@ template <typename Number> struct VmultParams {
   const double *__restrict__ coords;  // scalar / residual: node coordinates
   const Number *__restrict__ mu;      // per sorted cell
   const Number *__restrict__ lambda;
-  int cell_begin, cell_end;
+  int cell_begin, cell_end;  // sorted cells of one colour, or its tiles (tiled layout)
+  int formulation;  // 1, or 0: the volumetric/isochoric split (scalar, tensor4, residual)
+  const uint32_t *__restrict__ tnodes;  // tiled layout: [tile][TN] node codes
+  const int32_t *__restrict__ tinfo;    // tiled layout: [tile][2] first cell, extents
   // adjoint: per-cell outputs in mesh order (perm: sorted -> mesh cell)
   double *__restrict__ dmu;
   double *__restrict__ dlambda;
@@ -488,24 +653,77 @@ __device__ __forceinline__ void kinematics_prologue(const VmultParams<T> &P, con
   lds_barrier();  // all lanes done reading A/B/Xc of this prologue
 }
 
+// scalar_referential kinematics (include/mf_nh_operator.h:1276-1310): the
+// displacement at the Gauss points comes from the cache (uq), its reference
+// gradient from the collocation derivative on the Gauss points (no gather of
+// u); J_ref from the corner coordinates.  Leaves A free.
+template <int DIM, int N, typename T>
+__device__ __forceinline__ void kinematics_collocation(const VmultParams<T> &P, const T *tab, T *A, T *Xc, bool on,
+                                                       bool active, uint32_t code, int r, int i0, int i1, int i2,
+                                                       const T (&uq)[DIM], T (&F)[DIM][DIM], T (&Jri)[DIM][DIM],
+                                                       T &detJr) {
+  constexpr int NQP = ipow(N, DIM);
+  const int node = code & kNodeMask;
+  if (on) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) A[c * NQP + r] = uq[c];
+  }
+  if (active) {
+    const int v = node_corner<DIM, N>(i0, i1, i2);
+    if (v >= 0)
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) Xc[v * DIM + c] = T(P.coords[(size_t)node * DIM + c]);
+  }
+  lds_barrier();
+  const T *Dc = tab + 4 * N * N + 2 * N;
+  const int ii[3] = {i0, i1, i2};
+  const int stride[3] = {1, N, N * N};
+  T Gu[DIM][DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int base = r - ii[d] * stride[d];  // in [0, NQP) for every lane
+      T s = T(0);
+#pragma unroll
+      for (int k = 0; k < N; ++k) s = fma(Dc[ii[d] * N + k], A[c * NQP + base + k * stride[d]], s);
+      Gu[c][d] = s;
+    }
+  T Jr[DIM][DIM];
+  q1_jacobian<DIM, T, T>(Xc, tab + 4 * N * N, i0, i1, i2, Jr);
+  detJr = inverse<DIM>(Jr, Jri);
+  grad_times<DIM>(Gu, Jri, F);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) F[d][d] += T(1);
+  lds_barrier();  // all lanes done reading A/Xc
+}
+
 // ------------------------------------------------------------------------
 // fused vmult kernel, one launch per colour
-//   VAR: V_SCALAR / V_TENSOR2 / V_TENSOR4 / V_RESIDUAL
+//   VAR: V_SCALAR / V_SCALAR_REF / V_TENSOR2 / V_TENSOR4 / V_TENSOR4_NS /
+//        V_RESIDUAL / V_ADJOINT
 //   ADD: dst += A src for every touch; otherwise the first-touch cell of each
 //        DoF stores (no separate zeroing pass)
 //   CONSTR: constrained DoFs read as zero and are skipped on write
 //   PH: phase bits (SF, QD) for breakdown timings
+//   TILE: tiled layout -- one workgroup per tile of a structured mesh; the
+//        tile's nodes are gathered once into LDS and the cells' results are
+//        summed there (fixed order) before one read-modify-write per node
 // ------------------------------------------------------------------------
-template <int DIM, int N, typename T, int VAR, bool ADD, bool CONSTR, unsigned PH>
+template <int DIM, int N, typename T, int VAR, bool ADD, bool CONSTR, unsigned PH, bool TILE>
 __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   using CS = CellShape<DIM, N>;
+  using TS = TileShape<DIM, N>;
   constexpr int NQP = CS::NQP, CPW = CS::CPW, NB = CS::NB, NC = CS::NC;
+  constexpr int TN = TILE ? TS::TN : 1;
   constexpr bool RECOMP = (VAR == V_RESIDUAL || VAR == V_ADJOINT);  // no cache, c from ln J in FP64
   constexpr int NF = CacheFields<DIM, (RECOMP ? V_SCALAR : VAR)>::value;
-  constexpr bool KIN = (VAR == V_SCALAR || RECOMP);
-  constexpr int TABN = 4 * N * N + 2 * N;
+  constexpr bool KIN = (VAR == V_SCALAR || VAR == V_SCALAR_REF || RECOMP);
+  constexpr int TABN = tab_size<N>(VAR);
   constexpr int RED = (VAR == V_ADJOINT) ? CPW * 2 * NQP : 0;  // per-cell reductions
   __shared__ T lds[CPW * 2 * NB * NQP + TABN + (KIN ? CPW * NC * DIM : 0) + RED];
+  __shared__ T tile_v[TILE ? TN * DIM : 1];         // tile nodes' source values
+  __shared__ uint32_t tile_c[TILE ? TN : 1];        // tile nodes' codes
   T *tab = lds + CPW * 2 * NB * NQP;
   T *Xall = tab + TABN;
   T *Rall = Xall + (KIN ? CPW * NC * DIM : 0);
@@ -516,9 +734,7 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   const int r = t - slot * NQP;
   const int i0 = r % N, i1 = (r / N) % N, i2 = (DIM == 3) ? r / (N * N) : 0;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int cell = P.cell_begin + L * CPW + slot;
   const bool on = slot < CPW;
-  const bool active = on && (cell < P.cell_end);
   T *A = lds + (slot < CPW ? slot : 0) * 2 * NB * NQP;
   T *B = A + NB * NQP;
   T *Xc = Xall + (slot < CPW ? slot : 0) * NC * DIM;
@@ -529,15 +745,43 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   // in flight through the sum-factorization stages (LDS-only barriers).
   // The loads are branch-free (idle lanes read a valid cell of this colour
   // and discard the values) so the wait counts stay static.
-  const size_t cell_s = active ? (size_t)cell : (size_t)P.cell_begin;
-  const int r_s = on ? r : 0;
-  const uint32_t code = P.idx[cell_s * NQP + r_s];
-  const size_t node = code & kNodeMask;
+  int cell;
+  bool active;
+  size_t cell_s;
+  uint32_t code = 0;
+  size_t node = 0;
+  // tiled layout: slot -> (sx, sy, sz) in the tile, lane t < TN -> tile node
+  const int sx = slot % TS::TX, sy = (slot / TS::TX) % TS::TY, sz = slot / (TS::TX * TS::TY);
+  const int tile = P.cell_begin + L;
+  uint32_t tcode = 0;
+  bool tvalid = false;
+  size_t tnode = 0;
   T sv[DIM];
-  if constexpr (!KIN) {
+  if constexpr (TILE) {
+    const int first = P.tinfo[2 * (size_t)tile], ext = P.tinfo[2 * (size_t)tile + 1];
+    const int ex = ext & 0xff, ey = (ext >> 8) & 0xff, ez = ext >> 16;
+    active = on && sx < ex && sy < ey && sz < ez;
+    cell = first + sx + ex * (sy + ey * sz);
+    cell_s = active ? (size_t)cell : (size_t)first;
+    tcode = P.tnodes[(size_t)tile * TN + (t < TN ? t : 0)];
+    tvalid = t < TN && tcode != kNoNode;
+    tnode = tvalid ? (tcode & kNodeMask) : 0;
+    if constexpr (VAR != V_RESIDUAL) {
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) sv[c] = P.src[node * DIM + c];
+      for (int c = 0; c < DIM; ++c) sv[c] = P.src[tnode * DIM + c];
+    }
+  } else {
+    cell = P.cell_begin + L * CPW + slot;
+    active = on && (cell < P.cell_end);
+    cell_s = active ? (size_t)cell : (size_t)P.cell_begin;
+    code = P.idx[cell_s * NQP + (on ? r : 0)];
+    node = code & kNodeMask;
+    if constexpr (!KIN) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) sv[c] = P.src[node * DIM + c];
+    }
   }
+  const int r_s = on ? r : 0;
   __builtin_amdgcn_sched_barrier(0);
   T cch[NF > 0 ? NF : 1];
   if constexpr ((PH & PH_QD) && NF > 0 && !RECOMP) {
@@ -547,8 +791,31 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   }
   __builtin_amdgcn_sched_barrier(0);
 
+  // tile-local index of this lane's node (tiled layout)
+  const int ln = on ? (sx * (N - 1) + i0) + TS::NX * ((sy * (N - 1) + i1) + TS::NY * (sz * (N - 1) + i2)) : 0;
+  if constexpr (TILE) {
+    if (t < TN) {
+      tile_c[t] = tcode;
+      if constexpr (VAR != V_RESIDUAL) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+          const bool zero = !tvalid || (CONSTR && ((tcode >> (kConsShift + c)) & 1u));
+          tile_v[t * DIM + c] = zero ? T(0) : sv[c];
+        }
+      }
+    }
+    lds_barrier();
+    code = tile_c[ln];
+    node = code & kNodeMask;
+  }
+
   T F[DIM][DIM], Jri[DIM][DIM], detJr = T(1);
-  if constexpr (KIN && (PH & PH_QD)) {
+  if constexpr (VAR == V_SCALAR_REF && (PH & PH_QD)) {
+    T uq[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) uq[c] = cch[1 + c];
+    kinematics_collocation<DIM, N, T>(P, tab, A, Xc, on, active, code, r, i0, i1, i2, uq, F, Jri, detJr);
+  } else if constexpr (KIN && (PH & PH_QD)) {
     kinematics_prologue<DIM, N, T>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr);
   }
 
@@ -556,7 +823,11 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   T out[DIM];
   if constexpr (VAR != V_RESIDUAL) {
     // gather src with constraints (read_dof_values, include/mf_nh_operator.h:745-749)
-    if (active) {
+    if constexpr (TILE) {
+      if (on)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) A[c * NQP + r] = tile_v[ln * DIM + c];
+    } else if (active) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
         T v;
@@ -584,11 +855,14 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
         for (int d = 0; d < DIM; ++d) Phi[c][d] = G[c][d];
     } else if constexpr (VAR == V_TENSOR4) {
       qp_tensor4<DIM, T>(G, cch, Phi);
+    } else if constexpr (VAR == V_TENSOR4_NS) {
+      qp_tensor4_ns<DIM, T>(G, cch, Phi);
     } else if constexpr (VAR == V_TENSOR2) {
       qp_tensor2<DIM, T>(G, cch, Phi);
     } else {
-      // scalar (include/mf_nh_operator.h:1182-1254) or residual
-      // (include/mf_elasticity.h:2769-2804); kinematics recomputed here
+      // scalar (include/mf_nh_operator.h:1182-1254), scalar_referential
+      // (:1255-1400) or residual (include/mf_elasticity.h:2769-2804);
+      // kinematics recomputed here
       const T m = active ? P.mu[cell] : T(0), l = active ? P.lambda[cell] : T(0);
       T Fi[DIM][DIM];
       const T J = inverse<DIM>(F, Fi);
@@ -598,6 +872,34 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
       const T dw = detJr * w;
       T K[DIM][DIM];  // J_cur^-1 = J_ref^-1 F^-1
       grad_times<DIM>(Jri, Fi, K);
+      if (P.formulation == 0 && VAR != V_ADJOINT) {
+        // formulation 0 (include/mf_nh_operator.h:1005-1180): the scalar
+        // cache holds J^{-1/d}; the residual evaluates it here
+        T sbar;
+        if constexpr (RECOMP) sbar = T(pow(double(J), -1.0 / DIM));
+        else sbar = active ? cch[0] : T(1);
+        const Form0<DIM, T> f0(F, J, sbar, m, l);
+        if constexpr (VAR == V_RESIDUAL) {
+#pragma unroll
+          for (int c = 0; c < DIM; ++c)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+              T s = f0.tau[c][0] * K[d][0];
+#pragma unroll
+              for (int e = 1; e < DIM; ++e) s = fma(f0.tau[c][e], K[d][e], s);
+              Phi[c][d] = -s * dw;
+            }
+        } else {
+          T g[DIM][DIM], Q[DIM][DIM];
+          grad_times<DIM>(G, K, g);
+          f0.action(g, Q);
+#pragma unroll
+          for (int i = 0; i < DIM; ++i)
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) Q[i][j] *= dw;
+          flux_times_kt<DIM>(Q, K, Phi);
+        }
+      } else {
       T tau[DIM][DIM];
       T cs;
       if constexpr (RECOMP) cs = m - T(2) * l * T(log(double(J)));
@@ -666,6 +968,7 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
           }
         }
       }
+      }  // formulation 1
     }
     if constexpr (DIM == 3)
       SF<DIM, N, T>::integrate(tab, B, A, on, r, i0, i1, i2, Phi, out);
@@ -687,6 +990,50 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
     }
   }
 
+  if constexpr (TILE) {
+    // sum the cells' results per tile node in LDS (fixed order), then one
+    // read-modify-write per node (distribute_local_to_global, :769-780)
+    lds_barrier();  // every lane is done with the sum-factorization buffers
+    T *res = lds;   // CPW * NQP * DIM values, within those buffers
+    if (on)
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) res[t * DIM + c] = out[c];
+    lds_barrier();
+    if (tvalid) {
+      const int first = P.tinfo[2 * (size_t)tile], ext = P.tinfo[2 * (size_t)tile + 1];
+      (void)first;
+      const int ex = ext & 0xff, ey = (ext >> 8) & 0xff, ez = ext >> 16;
+      const int X = t % TS::NX, Y = (t / TS::NX) % TS::NY, Z = t / (TS::NX * TS::NY);
+      constexpr int PP = N - 1;
+      T acc[DIM];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) acc[c] = T(0);
+      const int zlo = Z == 0 ? 0 : (Z - 1) / PP, zhi = min(Z / PP, ez - 1);
+      const int ylo = Y == 0 ? 0 : (Y - 1) / PP, yhi = min(Y / PP, ey - 1);
+      const int xlo = X == 0 ? 0 : (X - 1) / PP, xhi = min(X / PP, ex - 1);
+      for (int cz = zlo; cz <= zhi; ++cz)
+        for (int cy = ylo; cy <= yhi; ++cy)
+          for (int cx = xlo; cx <= xhi; ++cx) {
+            const int rr = (X - cx * PP) + N * ((Y - cy * PP) + N * (Z - cz * PP));
+            const int lane = (cx + TS::TX * (cy + TS::TY * cz)) * NQP + rr;
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) acc[c] += res[lane * DIM + c];
+          }
+      const bool first_touch = !ADD && (tcode & kFirstBit);
+      T old[DIM];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        const bool skip = CONSTR && ((tcode >> (kConsShift + c)) & 1u);
+        old[c] = (first_touch || skip) ? T(0) : P.dst[tnode * DIM + c];
+      }
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        if (CONSTR && ((tcode >> (kConsShift + c)) & 1u)) continue;
+        P.dst[tnode * DIM + c] = old[c] + acc[c];
+      }
+    }
+    return;
+  }
   // colour-ordered scatter (distribute_local_to_global, :769-780): all reads
   // of the node's values are issued before the writes (one round trip)
   if (active) {
@@ -730,6 +1077,7 @@ struct CacheParams {
   void *cache;
   size_t fs;
   int n_cells;
+  int formulation;
   int *bad;
 };
 
@@ -745,6 +1093,7 @@ template <int DIM, int N, typename T, int VAR> __global__ void __launch_bounds__
   const int slot = t / NQP, r = t - slot * NQP;
   const int i0 = r % N, i1 = (r / N) % N, i2 = (DIM == 3) ? r / (N * N) : 0;
   const int cell = blockIdx.x * CPW + slot;
+  constexpr int D2 = DIM * DIM;
   const bool on = slot < CPW;
   const bool active = on && (cell < P.n_cells);
   double *A = lds + (slot < CPW ? slot : 0) * 2 * NB * NQP;
@@ -753,10 +1102,32 @@ template <int DIM, int N, typename T, int VAR> __global__ void __launch_bounds__
   uint32_t code = 0;
   if (active) code = P.idx[(size_t)cell * NQP + r];
   VmultParams<double> KP{};
+  KP.formulation = P.formulation;
   KP.u = P.u;
   KP.coords = P.coords;
   double F[DIM][DIM], Jri[DIM][DIM], detJr;
   kinematics_prologue<DIM, N, double>(KP, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr);
+  double uq[DIM];
+  if constexpr (VAR == V_SCALAR_REF) {
+    // u at this Gauss point: the tensor-product interpolation of the nodal
+    // values (phi_reference values, include/mf_nh_operator.h:527-533)
+    const int node = code & kNodeMask;
+    if (active)
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) A[c * NQP + r] = P.u[(size_t)node * DIM + c];
+    lds_barrier();
+    const double *S = tab;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) uq[c] = 0.0;
+    if (on)
+      for (int a = 0; a < NQP; ++a) {
+        const int a0 = a % N, a1 = (a / N) % N, a2 = (DIM == 3) ? a / (N * N) : 0;
+        double ph = S[i0 * N + a0] * S[i1 * N + a1];
+        if (DIM == 3) ph *= S[i2 * N + a2];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) uq[c] = fma(ph, A[c * NQP + a], uq[c]);
+      }
+  }
   if (!active) return;
   const double m = P.mu[cell], l = P.lambda[cell];
   double Fi[DIM][DIM];
@@ -773,8 +1144,39 @@ template <int DIM, int N, typename T, int VAR> __global__ void __launch_bounds__
   T *out = static_cast<T *>(P.cache) + (size_t)cell * NQP + r;
   auto put = [&](int f, double v) { out[(size_t)f * P.fs] = T(v); };
   if constexpr (VAR == V_SCALAR) {
+    // formulation 0 caches J^{-1/d} (include/mf_nh_operator.h:467-487)
+    put(0, P.formulation == 0 ? pow(J, -1.0 / DIM) : cs);
+  } else if constexpr (VAR == V_SCALAR_REF) {
     put(0, cs);
-  } else {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) put(1 + c, uq[c]);
+  } else if constexpr (VAR == V_TENSOR4_NS) {
+    // dP/dF_(iJ)(kL) = mu d_ik d_JL + c Fi_Jk Fi_Li + 2 lambda Fi_Ji Fi_Lk
+    // (include/mf_nh_operator.h:563-582), times det(J_ref) w
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) put(d * DIM + j, Jri[d][j]);
+#pragma unroll
+    for (int p = 0; p < D2; ++p)
+#pragma unroll
+      for (int q = p; q < D2; ++q) {
+        const int i = p / DIM, Jc = p % DIM, k = q / DIM, L = q % DIM;
+        const double a = (i == k && Jc == L ? m : 0.0) + cs * Fi[Jc][k] * Fi[L][i] + 2.0 * l * Fi[Jc][i] * Fi[L][k];
+        put(D2 + sym_index<D2>(p, q), a * dw);
+      }
+  } else if constexpr (VAR == V_TENSOR4) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) put(d * DIM + j, K[d][j]);
+    double Tv[NV], M[NV * (NV + 1) / 2];
+    tensor4_fields<DIM, double>(F, J, m, l, dw, P.formulation, Tv, M);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) put(D2 + v, Tv[v]);
+#pragma unroll
+    for (int k = 0; k < NV * (NV + 1) / 2; ++k) put(D2 + NV + k, M[k]);
+  } else {  // tensor2 (include/mf_nh_operator.h:535-547)
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
 #pragma unroll
@@ -788,21 +1190,8 @@ template <int DIM, int N, typename T, int VAR> __global__ void __launch_bounds__
       const double tau = m * s - (i == j ? cs : 0.0);
       put(DIM * DIM + v, tau * dw);
     }
-    if constexpr (VAR == V_TENSOR2) {
-      put(DIM * DIM + NV, 2.0 * cs * dw);
-      put(DIM * DIM + NV + 1, 2.0 * l * dw);
-    } else {
-#pragma unroll
-      for (int v = 0; v < NV; ++v)
-#pragma unroll
-        for (int w2 = v; w2 < NV; ++w2) {
-          const int i = voigt_first<DIM>(v), j = voigt_second<DIM>(v);
-          const int k = voigt_first<DIM>(w2), ll = voigt_second<DIM>(w2);
-          const double Sijkl = 0.5 * (double(i == k && j == ll) + double(i == ll && j == k));
-          const double IxI = (i == j && k == ll) ? 1.0 : 0.0;
-          put(DIM * DIM + NV + sym_index<NV>(v, w2), (2.0 * cs * Sijkl + 2.0 * l * IxI) * dw);
-        }
-    }
+    put(DIM * DIM + NV, 2.0 * cs * dw);
+    put(DIM * DIM + NV + 1, 2.0 * l * dw);
   }
 }
 
@@ -815,27 +1204,48 @@ template <int DIM, int N, typename T, int VAR> __global__ void __launch_bounds__
 //                          + c2 gamma_c^2               (tensor2 / scalar)
 // where s~ is the Voigt vector of sym(e_c (x) gamma) with doubled shears.
 // ------------------------------------------------------------------------
-template <int DIM, int N, typename T, int VAR>
+template <int DIM, int N, typename T, int VAR, bool TILE>
 __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
   using CS = CellShape<DIM, N>;
+  using TS = TileShape<DIM, N>;
   constexpr int NQP = CS::NQP, CPW = CS::CPW, NB = CS::NB, NC = CS::NC;
   constexpr int NV = Voigt<DIM>::NV;
-  constexpr int NF = (VAR == V_TENSOR4) ? CacheFields<DIM, V_TENSOR4>::value : CacheFields<DIM, V_TENSOR2>::value;
+  // the scalar variant evaluates the tensor4 fields per point (either
+  // formulation) and shares the tensor4 closed form
+  constexpr int NF = (VAR == V_TENSOR2) ? CacheFields<DIM, V_TENSOR2>::value
+                     : (VAR == V_TENSOR4_NS) ? CacheFields<DIM, V_TENSOR4_NS>::value
+                                             : CacheFields<DIM, V_TENSOR4>::value;
   constexpr int TABN = 4 * N * N + 2 * N;
   constexpr int SFN = (VAR == V_SCALAR) ? CPW * 2 * NB * NQP : 0;
-  __shared__ T lds[CPW * NF * NQP + TABN + SFN + (VAR == V_SCALAR ? CPW * NC * DIM : 0)];
+  constexpr int XN = (VAR == V_SCALAR) ? CPW * NC * DIM : 0;
+  // the kinematic prologue's scratch (SF ping-pong, corners) is dead once the
+  // per-point fields are written, so it shares the fields' region
+  constexpr int QDN = CPW * NF * NQP > SFN + XN ? CPW * NF * NQP : SFN + XN;
+  __shared__ T lds[QDN + TABN];
   T *qd_all = lds;
-  T *tab = lds + CPW * NF * NQP;
-  T *sf_all = tab + TABN;
+  T *tab = lds + QDN;
+  T *sf_all = lds;
   T *X_all = sf_all + SFN;
   const int t = threadIdx.x;
   if (t < TABN) tab[t] = P.tab[t];
   const int slot = t / NQP, r = t - slot * NQP;
   const int i0 = r % N, i1 = (r / N) % N, i2 = (DIM == 3) ? r / (N * N) : 0;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int cell = P.cell_begin + L * CPW + slot;
   const bool on = slot < CPW;
-  const bool active = on && (cell < P.cell_end);
+  const int tile = P.cell_begin + L;
+  int cell;
+  bool active;
+  int ex = 0, ey = 0, ez = 0;
+  if constexpr (TILE) {
+    const int first = P.tinfo[2 * (size_t)tile], ext = P.tinfo[2 * (size_t)tile + 1];
+    ex = ext & 0xff, ey = (ext >> 8) & 0xff, ez = ext >> 16;
+    const int sx = slot % TS::TX, sy = (slot / TS::TX) % TS::TY, sz = slot / (TS::TX * TS::TY);
+    active = on && sx < ex && sy < ey && sz < ez;
+    cell = first + sx + ex * (sy + ey * sz);
+  } else {
+    cell = P.cell_begin + L * CPW + slot;
+    active = on && (cell < P.cell_end);
+  }
   const int sl = slot < CPW ? slot : 0;
   T *qd = qd_all + sl * NF * NQP;
   uint32_t code = 0;
@@ -848,27 +1258,22 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
     kinematics_prologue<DIM, N, T>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr);
     if (active) {
       const T m = P.mu[cell], l = P.lambda[cell];
-      const T cs = P.cache[(size_t)cell * NQP + r];
       T Fi[DIM][DIM], K[DIM][DIM];
-      inverse<DIM>(F, Fi);
+      const T J = inverse<DIM>(F, Fi);
       grad_times<DIM>(Jri, Fi, K);
       const T *qw = tab + 4 * N * N + N;
       const T w = (DIM == 3) ? qw[i0] * qw[i1] * qw[i2] : qw[i0] * qw[i1];
       const T dw = detJr * w;
+      T Tv[NV], M[NV * (NV + 1) / 2];
+      tensor4_fields<DIM, T>(F, J, m, l, dw, P.formulation, Tv, M);
 #pragma unroll
       for (int d = 0; d < DIM; ++d)
 #pragma unroll
         for (int j = 0; j < DIM; ++j) qd[(d * DIM + j) * NQP + r] = K[d][j];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        const int i = voigt_first<DIM>(v), j = voigt_second<DIM>(v);
-        T s = T(0);
+      for (int v = 0; v < NV; ++v) qd[(DIM * DIM + v) * NQP + r] = Tv[v];
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) s += F[i][k] * F[j][k];
-        qd[(DIM * DIM + v) * NQP + r] = (m * s - (i == j ? cs : T(0))) * dw;
-      }
-      qd[(DIM * DIM + NV) * NQP + r] = T(2) * cs * dw;
-      qd[(DIM * DIM + NV + 1) * NQP + r] = T(2) * l * dw;
+      for (int k = 0; k < NV * (NV + 1) / 2; ++k) qd[(DIM * DIM + NV + k) * NQP + r] = M[k];
     }
   } else {
     if (active) {
@@ -878,7 +1283,7 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
     }
   }
   lds_barrier();
-  if (!active) return;
+  if (!TILE && !active) return;  // the tiled layout's lanes all reach the sum below
   const T *S = tab, *D = tab + N * N;
   const int ai[3] = {i0, i1, i2};
   T acc[DIM];
@@ -903,12 +1308,27 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
       for (int d = 0; d < DIM; ++d) s = fma(gph[d], qd[(d * DIM + j) * NQP + q], s);
       gam[j] = s;
     }
+    if constexpr (VAR == V_TENSOR4_NS) {
+      // gam = J_ref^-T grad_xi phi_a: A_(a,c),(a,c) = sum_q gam_e A_(c e)(c f) gam_f
+      constexpr int D2 = DIM * DIM;
+      const T *Am = qd + D2 * NQP;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        T s = T(0);
+#pragma unroll
+        for (int e = 0; e < DIM; ++e)
+#pragma unroll
+          for (int f = 0; f < DIM; ++f) s = fma(gam[e] * gam[f], Am[sym_index<D2>(c * DIM + e, c * DIM + f) * NQP + q], s);
+        acc[c] += s;
+      }
+      continue;
+    }
     T geo = T(0);
 #pragma unroll
     for (int i = 0; i < DIM; ++i)
 #pragma unroll
       for (int j = 0; j < DIM; ++j) geo = fma(gam[i] * gam[j], qd[(DIM * DIM + voigt_index<DIM>(i, j)) * NQP + q], geo);
-    if constexpr (VAR == V_TENSOR4) {
+    if constexpr (VAR == V_TENSOR4 || VAR == V_SCALAR) {
       const T *M = qd + (DIM * DIM + NV) * NQP;
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
@@ -933,6 +1353,44 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) acc[c] += geo + T(0.5) * c1 * (g2 + gam[c] * gam[c]) + c2 * gam[c] * gam[c];
     }
+  }
+  if constexpr (TILE) {
+    lds_barrier();  // every lane is done with the per-point fields
+    T *res = lds;   // CPW * NQP * DIM values, within the fields' region
+    if (on)
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) res[t * DIM + c] = active ? acc[c] : T(0);
+    lds_barrier();
+    if (t < TS::TN) {
+      const uint32_t tcode = P.tnodes[(size_t)tile * TS::TN + t];
+      if (tcode != kNoNode) {
+        const int X = t % TS::NX, Y = (t / TS::NX) % TS::NY, Z = t / (TS::NX * TS::NY);
+        constexpr int PP = N - 1;
+        T sum[DIM];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) sum[c] = T(0);
+        const int zlo = Z == 0 ? 0 : (Z - 1) / PP, zhi = min(Z / PP, ez - 1);
+        const int ylo = Y == 0 ? 0 : (Y - 1) / PP, yhi = min(Y / PP, ey - 1);
+        const int xlo = X == 0 ? 0 : (X - 1) / PP, xhi = min(X / PP, ex - 1);
+        for (int cz = zlo; cz <= zhi; ++cz)
+          for (int cy = ylo; cy <= yhi; ++cy)
+            for (int cx = xlo; cx <= xhi; ++cx) {
+              const int rr = (X - cx * PP) + N * ((Y - cy * PP) + N * (Z - cz * PP));
+              const int lane = (cx + TS::TX * (cy + TS::TY * cz)) * NQP + rr;
+#pragma unroll
+              for (int c = 0; c < DIM; ++c) sum[c] += res[lane * DIM + c];
+            }
+        const size_t tnode = tcode & kNodeMask;
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+          if ((tcode >> (kConsShift + c)) & 1u) continue;
+          T *d = P.dst + tnode * DIM + c;
+          if (tcode & kFirstBit) *d = sum[c];
+          else *d += sum[c];
+        }
+      }
+    }
+    return;
   }
   const size_t node = code & kNodeMask;
 #pragma unroll

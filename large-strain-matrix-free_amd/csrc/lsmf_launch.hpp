@@ -21,8 +21,12 @@ template <int DIM, int N> struct Launch {
     P.coords = op->d_coords;
     P.mu = static_cast<const T *>(op->d_mu);
     P.lambda = static_cast<const T *>(op->d_lambda);
-    P.cell_begin = op->colour_ptr[k];
-    P.cell_end = op->colour_ptr[k + 1];
+    const std::vector<int> &ptr = op->tiled ? op->tile_ptr : op->colour_ptr;
+    P.cell_begin = ptr[k];
+    P.cell_end = ptr[k + 1];
+    P.formulation = op->formulation;
+    P.tnodes = op->d_tile_nodes;
+    P.tinfo = op->d_tile_info;
     P.dmu = op->adj_dmu;
     P.dlambda = op->adj_dlambda;
     P.perm = op->d_perm;
@@ -30,10 +34,14 @@ template <int DIM, int N> struct Launch {
   }
 
   static int grid_for(int n) { return (n + CS::CPW - 1) / CS::CPW; }
+  static int n_colours(const lsmf_op *op) {
+    return (int)(op->tiled ? op->tile_ptr.size() : op->colour_ptr.size()) - 1;
+  }
 
+  // one launch per colour; the tiled layout launches one workgroup per tile
   template <typename T, int VAR, bool ADD, bool CONSTR, unsigned PH>
   static void run_colours(const lsmf_op *op, T *dst, const T *src, hipStream_t s) {
-    const int nc = (int)op->colour_ptr.size() - 1;
+    const int nc = n_colours(op);
     for (int k = 0; k < nc; ++k) {
       VmultParams<T> P = params<T>(op, k);
       P.src = src;
@@ -41,7 +49,11 @@ template <int DIM, int N> struct Launch {
       const int n = P.cell_end - P.cell_begin;
       if (n <= 0) continue;
       if (op->prof) op->prof->before(s);
-      hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, ADD, CONSTR, PH>), dim3(grid_for(n)), dim3(kWG), 0, s, P);
+      if (op->tiled)
+        hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, ADD, CONSTR, PH, true>), dim3(n), dim3(kWG), 0, s, P);
+      else
+        hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, ADD, CONSTR, PH, false>), dim3(grid_for(n)), dim3(kWG), 0, s,
+                           P);
       if (op->prof) op->prof->after(s);
     }
   }
@@ -84,6 +96,12 @@ template <int DIM, int N> struct Launch {
       case LSMF_CACHING_SCALAR:
         return f64 ? vmult_t<double, V_SCALAR>(op, dst, src, add, constr, ph, s)
                    : vmult_t<float, V_SCALAR>(op, dst, src, add, constr, ph, s);
+      case LSMF_CACHING_TENSOR4_NS:
+        return f64 ? vmult_t<double, V_TENSOR4_NS>(op, dst, src, add, constr, ph, s)
+                   : vmult_t<float, V_TENSOR4_NS>(op, dst, src, add, constr, ph, s);
+      case LSMF_CACHING_SCALAR_REFERENTIAL:
+        return f64 ? vmult_t<double, V_SCALAR_REF>(op, dst, src, add, constr, ph, s)
+                   : vmult_t<float, V_SCALAR_REF>(op, dst, src, add, constr, ph, s);
       default:
         return fail(LSMF_ERR_UNKNOWN_CACHING, "vmult: caching variant not available on the GPU");
     }
@@ -100,6 +118,7 @@ template <int DIM, int N> struct Launch {
     P.cache = op->d_cache;
     P.fs = op->field_stride;
     P.n_cells = op->n_cells;
+    P.formulation = op->formulation;
     P.bad = op->d_bad;
     hipLaunchKernelGGL((k_cache<DIM, N, T, VAR>), dim3(grid_for(op->n_cells)), dim3(kWG), 0, s, P);
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
@@ -114,19 +133,26 @@ template <int DIM, int N> struct Launch {
         return f64 ? cache_t<double, V_TENSOR2>(op, s) : cache_t<float, V_TENSOR2>(op, s);
       case LSMF_CACHING_SCALAR:
         return f64 ? cache_t<double, V_SCALAR>(op, s) : cache_t<float, V_SCALAR>(op, s);
+      case LSMF_CACHING_TENSOR4_NS:
+        return f64 ? cache_t<double, V_TENSOR4_NS>(op, s) : cache_t<float, V_TENSOR4_NS>(op, s);
+      case LSMF_CACHING_SCALAR_REFERENTIAL:
+        return f64 ? cache_t<double, V_SCALAR_REF>(op, s) : cache_t<float, V_SCALAR_REF>(op, s);
       default:
         return fail(LSMF_ERR_UNKNOWN_CACHING, "cache: caching variant not available on the GPU");
     }
   }
 
   template <typename T, int VAR> static int diagonal_t(const lsmf_op *op, hipStream_t s) {
-    const int nc = (int)op->colour_ptr.size() - 1;
+    const int nc = n_colours(op);
     for (int k = 0; k < nc; ++k) {
       VmultParams<T> P = params<T>(op, k);
       P.dst = static_cast<T *>(op->d_diag);
       const int n = P.cell_end - P.cell_begin;
       if (n <= 0) continue;
-      hipLaunchKernelGGL((k_diagonal<DIM, N, T, VAR>), dim3(grid_for(n)), dim3(kWG), 0, s, P);
+      if (op->tiled)
+        hipLaunchKernelGGL((k_diagonal<DIM, N, T, VAR, true>), dim3(n), dim3(kWG), 0, s, P);
+      else
+        hipLaunchKernelGGL((k_diagonal<DIM, N, T, VAR, false>), dim3(grid_for(n)), dim3(kWG), 0, s, P);
     }
     T *diag = static_cast<T *>(op->d_diag);
     if (op->n_cons > 0)
@@ -145,7 +171,10 @@ template <int DIM, int N> struct Launch {
       case LSMF_CACHING_TENSOR2:
         return f64 ? diagonal_t<double, V_TENSOR2>(op, s) : diagonal_t<float, V_TENSOR2>(op, s);
       case LSMF_CACHING_SCALAR:
+      case LSMF_CACHING_SCALAR_REFERENTIAL:  // c is field 0 of both caches; F from u
         return f64 ? diagonal_t<double, V_SCALAR>(op, s) : diagonal_t<float, V_SCALAR>(op, s);
+      case LSMF_CACHING_TENSOR4_NS:
+        return f64 ? diagonal_t<double, V_TENSOR4_NS>(op, s) : diagonal_t<float, V_TENSOR4_NS>(op, s);
       default:
         return fail(LSMF_ERR_UNKNOWN_CACHING, "diagonal: caching variant not available on the GPU");
     }

@@ -47,7 +47,9 @@ int n_cache_fields(int dim, int caching) {
   switch (caching) {
     case LSMF_CACHING_TENSOR4: return dim * dim + nv + nv * (nv + 1) / 2;
     case LSMF_CACHING_TENSOR2: return dim * dim + nv + 2;
+    case LSMF_CACHING_TENSOR4_NS: return dim * dim + dim * dim * (dim * dim + 1) / 2;
     case LSMF_CACHING_SCALAR: return 1;
+    case LSMF_CACHING_SCALAR_REFERENTIAL: return 1 + dim;
     default: return -1;
   }
 }
@@ -77,7 +79,7 @@ template <typename T> int dalloc(T **p, size_t n, const char *what) {
 void free_op(lsmf_op *op) {
   void *ptrs[] = {op->d_idx,  op->d_coords,    op->d_tab,  op->d_tab64, op->d_mu64,     op->d_lambda64,
                   op->d_mu,   op->d_lambda,    op->d_cache, op->d_diag, op->d_inv_diag, op->d_cons,
-                  op->d_bad,  op->d_perm};
+                  op->d_bad,  op->d_perm,      op->d_tile_nodes, op->d_tile_info};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (op->d_u64) (void)hipFree(op->d_u64);
@@ -85,6 +87,135 @@ void free_op(lsmf_op *op) {
 }
 
 __global__ void k_zero_int(int *p) { *p = 0; }
+
+// The lexicographic structured numbering of lsmf_mesh_create:
+// node(ix, iy, iz) = ix + Nx (iy + Ny iz), cell (cx, cy, cz) = cx + nx (cy + ny cz),
+// local node a = a0 + N (a1 + N a2) at (cx p + a0, cy p + a1, cz p + a2).
+// Returns the cells per direction when cell_nodes is exactly that.
+bool structured_dims(const lsmf_mesh_desc *m, int dim, int p, int nqp, long long n[3]) {
+  const int N = p + 1;
+  const int32_t *c0 = m->cell_nodes;
+  if (c0[0] != 0 || c0[1] != 1) return false;
+  const long long Nx = c0[N];
+  if (Nx < N || (Nx - 1) % p) return false;
+  long long Ny, Nz = 1;
+  if (dim == 3) {
+    const long long NxNy = c0[N * N];
+    if (NxNy % Nx) return false;
+    Ny = NxNy / Nx;
+    if (m->n_nodes % (Nx * Ny)) return false;
+    Nz = m->n_nodes / (Nx * Ny);
+  } else {
+    if (m->n_nodes % Nx) return false;
+    Ny = m->n_nodes / Nx;
+  }
+  if ((Ny - 1) % p || (Nz - 1) % p || Ny < N || (dim == 3 && Nz < N)) return false;
+  n[0] = (Nx - 1) / p;
+  n[1] = (Ny - 1) / p;
+  n[2] = dim == 3 ? (Nz - 1) / p : 1;
+  if (n[0] * n[1] * n[2] != m->n_cells) return false;
+  for (long long cz = 0; cz < n[2]; ++cz)
+    for (long long cy = 0; cy < n[1]; ++cy)
+      for (long long cx = 0; cx < n[0]; ++cx) {
+        const int32_t *cn = c0 + (size_t)(cx + n[0] * (cy + n[1] * cz)) * nqp;
+        for (int a = 0; a < nqp; ++a) {
+          const int a0 = a % N, a1 = (a / N) % N, a2 = dim == 3 ? a / (N * N) : 0;
+          if (cn[a] != (cx * p + a0) + Nx * ((cy * p + a1) + Ny * (cz * p + a2))) return false;
+        }
+      }
+  return true;
+}
+
+struct TileDims {
+  int tx, ty, tz, nx, ny, nz, tn;
+};
+template <int D, int NN> TileDims tile_dims_of() {
+  using TS = TileShape<D, NN>;
+  return {TS::TX, TS::TY, TS::TZ, TS::NX, TS::NY, TS::NZ, TS::TN};
+}
+bool tile_dims(int dim, int N, TileDims &t) {
+  if (dim == 2 && N == 2) t = tile_dims_of<2, 2>();
+  else if (dim == 2 && N == 3) t = tile_dims_of<2, 3>();
+  else if (dim == 2 && N == 5) t = tile_dims_of<2, 5>();
+  else if (dim == 3 && N == 2) t = tile_dims_of<3, 2>();
+  else if (dim == 3 && N == 3) t = tile_dims_of<3, 3>();
+  else if (dim == 3 && N == 5) t = tile_dims_of<3, 5>();
+  else return false;
+  return true;
+}
+
+// Tiles of a structured mesh, coloured by the parity of their position so
+// that no two tiles of one colour share a node.  Fills perm (tile order, the
+// cells of a tile consecutive in slot order), tile_ptr, the tile node table
+// (first-touch bit: the node's first tile in colour order) and tile info.
+int build_tiles(lsmf_op *op, const lsmf_mesh_desc *mesh, const long long n[3], std::vector<uint32_t> &tnodes,
+                std::vector<int32_t> &tinfo) {
+  const int dim = op->dim, p = op->degree;
+  TileDims td;
+  if (!tile_dims(dim, op->N, td)) return fail(LSMF_ERR_UNSUPPORTED, "tiles: no tile shape");
+  const long long Nx = n[0] * p + 1, Ny = n[1] * p + 1, Nz = dim == 3 ? n[2] * p + 1 : 1;
+  const long long TXn = (n[0] + td.tx - 1) / td.tx, TYn = (n[1] + td.ty - 1) / td.ty,
+                  TZn = (n[2] + td.tz - 1) / td.tz;
+  const long long n_tiles = TXn * TYn * TZn;
+  const int n_col = 1 << dim;
+  std::vector<long long> order;
+  order.reserve(n_tiles);
+  op->tile_ptr.assign(n_col + 1, 0);
+  for (int col = 0; col < n_col; ++col) {
+    for (long long bz = 0; bz < TZn; ++bz)
+      for (long long by = 0; by < TYn; ++by)
+        for (long long bx = 0; bx < TXn; ++bx)
+          if (((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2)) == col) order.push_back(bx + TXn * (by + TYn * bz));
+    op->tile_ptr[col + 1] = (int)order.size();
+  }
+  op->perm.assign(op->n_cells, 0);
+  op->inv_perm.assign(op->n_cells, 0);
+  tinfo.assign((size_t)n_tiles * 2, 0);
+  tnodes.assign((size_t)n_tiles * td.tn, kNoNode);
+  std::vector<int> min_col(mesh->n_nodes, 1 << 30), stamp(mesh->n_nodes, -1);
+  int pos = 0;
+  for (int col = 0; col < n_col; ++col)
+    for (int ti = op->tile_ptr[col]; ti < op->tile_ptr[col + 1]; ++ti) {
+      const long long b = order[ti];
+      const long long bx = b % TXn, by = (b / TXn) % TYn, bz = b / (TXn * TYn);
+      const int ex = (int)std::min<long long>(td.tx, n[0] - bx * td.tx);
+      const int ey = (int)std::min<long long>(td.ty, n[1] - by * td.ty);
+      const int ez = (int)std::min<long long>(td.tz, n[2] - bz * td.tz);
+      tinfo[(size_t)ti * 2] = pos;
+      tinfo[(size_t)ti * 2 + 1] = ex | (ey << 8) | (ez << 16);
+      for (int sz = 0; sz < ez; ++sz)
+        for (int sy = 0; sy < ey; ++sy)
+          for (int sx = 0; sx < ex; ++sx) {
+            const long long cell = (bx * td.tx + sx) + n[0] * ((by * td.ty + sy) + n[1] * (bz * td.tz + sz));
+            op->perm[pos] = (int)cell;
+            op->inv_perm[cell] = pos;
+            ++pos;
+          }
+      for (int l = 0; l < td.tn; ++l) {
+        const long long X = bx * td.tx * p + l % td.nx, Y = by * td.ty * p + (l / td.nx) % td.ny,
+                        Z = bz * td.tz * p + l / (td.nx * td.ny);
+        if (X >= Nx || Y >= Ny || Z >= Nz) continue;
+        const long long node = X + Nx * (Y + Ny * Z);
+        if (stamp[node] == col) return fail(LSMF_ERR_INVALID_ARGUMENT, "tiles: two tiles of one colour share a node");
+        stamp[node] = col;
+        min_col[node] = std::min(min_col[node], col);
+        tnodes[(size_t)ti * td.tn + l] = (uint32_t)node;
+      }
+    }
+  if (pos != op->n_cells) return fail(LSMF_ERR_INVALID_ARGUMENT, "tiles: cell count mismatch");
+  for (int col = 0; col < n_col; ++col)
+    for (int ti = op->tile_ptr[col]; ti < op->tile_ptr[col + 1]; ++ti)
+      for (int l = 0; l < td.tn; ++l) {
+        uint32_t &code = tnodes[(size_t)ti * td.tn + l];
+        if (code == kNoNode) continue;
+        const uint32_t node = code;
+        if (mesh->constrained)
+          for (int comp = 0; comp < dim; ++comp)
+            if (mesh->constrained[(size_t)node * dim + comp]) code |= 1u << (kConsShift + comp);
+        if (min_col[node] == col) code |= kFirstBit;
+      }
+  return LSMF_OK;
+}
 
 __global__ void k_convert_f64_f32(const double *__restrict__ a, float *__restrict__ b, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -138,7 +269,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   op->n_fields = n_cache_fields(dim, opt->caching);
   op->field_stride = (size_t)nc * nqp;
 
-  // --- colouring: given or greedy first fit
+  // --- colouring (given, checked below, or greedy first fit)
   std::vector<int> colour(nc, 0);
   if (mesh->cell_colour) {
     for (int c = 0; c < nc; ++c) {
@@ -163,6 +294,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       for (int a = 0; a < nqp; ++a) used[mesh->cell_nodes[(size_t)c * nqp + a]] |= (1ull << col);
     }
   }
+  std::vector<int> min_col;
   int n_col = 0;
   for (int c = 0; c < nc; ++c) n_col = std::max(n_col, colour[c] + 1);
   op->colour_ptr.assign(n_col + 1, 0);
@@ -179,7 +311,8 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
     }
   }
   // colour conflict check: no node twice inside one colour; min colour per node
-  std::vector<int> stamp(mesh->n_nodes, -1), min_col(mesh->n_nodes, 1 << 30);
+  std::vector<int> stamp(mesh->n_nodes, -1);
+  min_col.assign(mesh->n_nodes, 1 << 30);
   for (int k = 0; k < n_col; ++k)
     for (int pos = op->colour_ptr[k]; pos < op->colour_ptr[k + 1]; ++pos) {
       const int c = op->perm[pos];
@@ -193,6 +326,19 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
         min_col[nd] = std::min(min_col[nd], k);
       }
     }
+  // --- cell order of the tiled layout (structured meshes) replaces the
+  // colour order; the per-cell index table then carries no first-touch bits
+  std::vector<uint32_t> tnodes;
+  std::vector<int32_t> tinfo;
+  long long sn[3] = {1, 1, 1};
+  op->tiled = opt->layout == LSMF_LAYOUT_AUTO && structured_dims(mesh, dim, p, nqp, sn);
+  if (op->tiled) {
+    const int rc = build_tiles(op, mesh, sn, tnodes, tinfo);
+    if (rc) {
+      delete op;
+      return rc;
+    }
+  }
   // --- index table (sorted cells): node | constraint bits | first-touch bit
   std::vector<uint32_t> idx((size_t)nc * nqp);
   for (int pos = 0; pos < nc; ++pos) {
@@ -203,7 +349,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       if (mesh->constrained)
         for (int comp = 0; comp < dim; ++comp)
           if (mesh->constrained[(size_t)nd * dim + comp]) code |= 1u << (kConsShift + comp);
-      if (colour[c] == min_col[nd]) code |= kFirstBit;
+      if (!op->tiled && colour[c] == min_col[nd]) code |= kFirstBit;
       idx[(size_t)pos * nqp + a] = code;
     }
   }
@@ -213,10 +359,14 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       if (mesh->constrained[i]) cons.push_back((int32_t)i);
   op->n_cons = (int)cons.size();
 
-  // --- 1D tables: S, D, St, Dt, qp, qw
+  // --- 1D tables: S, D, St, Dt, qp, qw, and the collocation derivative on
+  // the Gauss points Dc[q][k] = l_k'(x_q) (l_k: Lagrange basis on the Gauss
+  // points; scalar_referential)
   Tables1D t1;
   t1.init(p, nq);
-  std::vector<double> tab(4 * N * N + 2 * N);
+  std::vector<double> tab(5 * N * N + 2 * N);
+  for (int q = 0; q < N; ++q)
+    for (int k = 0; k < N; ++k) tab[4 * N * N + 2 * N + q * N + k] = lagrange_derivative(t1.qpts, k, t1.qpts[q]);
   for (int q = 0; q < N; ++q)
     for (int a = 0; a < N; ++a) {
       tab[q * N + a] = t1.S[q * N + a];
@@ -243,7 +393,9 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       (rc = dalloc((char **)&op->d_diag, op->n_dofs * es, "diagonal")) ||
       (rc = dalloc((char **)&op->d_inv_diag, op->n_dofs * es, "inverse diagonal")) ||
       (rc = dalloc(&op->d_cons, cons.size(), "constrained list")) || (rc = dalloc(&op->d_bad, 1, "counter")) ||
-      (rc = dalloc(&op->d_perm, (size_t)nc, "cell permutation"))) {
+      (rc = dalloc(&op->d_perm, (size_t)nc, "cell permutation")) ||
+      (rc = dalloc(&op->d_tile_nodes, tnodes.size(), "tile nodes")) ||
+      (rc = dalloc(&op->d_tile_info, tinfo.size(), "tile info"))) {
     free_op(op);
     delete op;
     return rc;
@@ -272,6 +424,11 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   if (e == hipSuccess && !cons.empty())
     e = hipMemcpy(op->d_cons, cons.data(), cons.size() * sizeof(int32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(op->d_perm, op->perm.data(), (size_t)nc * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !tnodes.empty())
+    e = hipMemcpy(op->d_tile_nodes, tnodes.data(), tnodes.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !tinfo.empty())
+    e = hipMemcpy(op->d_tile_info, tinfo.data(), tinfo.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  op->bytes_idx += tnodes.size() * sizeof(uint32_t) + tinfo.size() * sizeof(int32_t);
   if (e != hipSuccess) {
     free_op(op);
     delete op;
@@ -308,6 +465,18 @@ lsmf_status lsmf_set_material_per_cell(lsmf_op *op, const double *mu, const doub
     LSMF_HIP_TRY(hipMemcpy(op->d_lambda, lf.data(), nc * sizeof(float), hipMemcpyHostToDevice), "set_material");
   }
   op->material_set = true;
+  op->cached = false;
+  op->diag_ok = false;
+  return LSMF_OK;
+}
+
+lsmf_status lsmf_set_formulation(lsmf_op *op, int formulation) {
+  if (!op) return fail(LSMF_ERR_INVALID_ARGUMENT, "set_formulation: null operator");
+  if (formulation != 0 && formulation != 1)
+    return fail(LSMF_ERR_INVALID_ARGUMENT, "set_formulation: Unknown material formulation");
+  if (formulation == 0 && op->caching != LSMF_CACHING_SCALAR && op->caching != LSMF_CACHING_TENSOR4)
+    return fail(LSMF_ERR_UNSUPPORTED, "set_formulation: formulation 0 needs scalar or tensor4 caching");
+  op->formulation = formulation;
   op->cached = false;
   op->diag_ok = false;
   return LSMF_OK;
@@ -401,7 +570,8 @@ size_t lsmf_memory_consumption(const lsmf_op *op) {
   if (!op) return 0;
   const size_t es = elem_size(op);
   size_t b = op->bytes_cache + op->bytes_idx + 2 * (size_t)op->n_cells * es;
-  if (op->caching == LSMF_CACHING_SCALAR) b += op->n_dofs * es + (size_t)op->n_nodes * op->dim * sizeof(double);
+  if (op->caching == LSMF_CACHING_SCALAR || op->caching == LSMF_CACHING_SCALAR_REFERENTIAL)
+    b += op->n_dofs * es + (size_t)op->n_nodes * op->dim * sizeof(double);
   return b;
 }
 
@@ -439,6 +609,7 @@ lsmf_status lsmf_residual_vjp(lsmf_op *op, const double *w_dev, double *du_dev, 
   if ((const void *)w_dev == (const void *)du_dev) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual_vjp: w and du alias");
   if (!op->material_set || !op->linearization_set)
     return fail(LSMF_ERR_INVALID_ARGUMENT, "residual_vjp: material and linearisation must be set");
+  if (op->formulation != 1) return fail(LSMF_ERR_UNSUPPORTED, "residual_vjp: formulation 1 only");
   op->adj_dmu = dmu;
   op->adj_dlambda = dlambda;
   const int rc = launch_table(op->dim, op->N)->residual_vjp(op, du_dev, w_dev, (hipStream_t)stream);

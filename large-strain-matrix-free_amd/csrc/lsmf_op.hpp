@@ -48,6 +48,7 @@ struct lsmf_op {
   // shape
   int dim = 0, degree = 0, nq = 0, N = 0, nqp = 0;
   int precision = LSMF_FP64, caching = LSMF_CACHING_TENSOR4;
+  int formulation = 1;  // material formulation (include/material.h:82-94)
   int n_cells = 0, n_nodes = 0;
   size_t n_dofs = 0;
   int n_fields = 0;
@@ -72,6 +73,13 @@ struct lsmf_op {
   int *d_bad = nullptr;
   int32_t *d_perm = nullptr;        // sorted position -> mesh cell
   double *adj_dmu = nullptr, *adj_dlambda = nullptr;  // outputs of the running residual VJP
+  // tiled layout (structured meshes): tiles sorted by colour, tiles of colour
+  // k are [tile_ptr[k], tile_ptr[k+1]); each tile's cells are consecutive
+  // sorted cells
+  bool tiled = false;
+  std::vector<int> tile_ptr;
+  uint32_t *d_tile_nodes = nullptr;  // [n_tiles][TN] node | constraint bits | first-touch bit, or kNoNode
+  int32_t *d_tile_info = nullptr;    // [n_tiles][2] first sorted cell, extents ex | ey << 8 | ez << 16
   bool material_set = false, linearization_set = false, cached = false, diag_ok = false;
   size_t bytes_idx = 0, bytes_cache = 0;
 };

@@ -317,6 +317,7 @@ template <typename T> class GMG : public GMGBase {
         lsmf_mesh_desc md{o.dim, o.degree, v.g->mesh->n_cells, v.g->mesh->n_nodes, v.g->mesh->cell_nodes,
                           v.g->mesh->coords, v.g->cons.data(), v.g->mesh->cell_colour};
         v.own->initialize(md, o.n_q_1d, o.mg_caching, s);
+        v.own->set_formulation(o.formulation);
         v.own->set_material(v.g->mu.data(), v.g->lam.data());
         v.op = v.own.get();
       }
@@ -363,6 +364,7 @@ template <typename T> class GMG : public GMGBase {
         lsmf_mesh_desc md{o.dim, o.degree, g0.mesh->n_cells, g0.mesh->n_nodes, g0.mesh->cell_nodes, g0.mesh->coords,
                           g0.cons.data(), g0.mesh->cell_colour};
         coarse64_own_->initialize(md, o.n_q_1d, o.mg_caching, s);
+        coarse64_own_->set_formulation(o.formulation);
         coarse64_own_->set_material(g0.mu.data(), g0.lam.data());
         coarse64_ = coarse64_own_.get();
       }
@@ -885,6 +887,7 @@ lsmf_status lsmf_solver_options_default(lsmf_solver_options *o) {
   o->use_graph = 1;
   o->preconditioner = LSMF_PRECONDITIONER_GMG;
   o->preconditioner_relaxation = 0.65;
+  o->formulation = 1;
   return LSMF_OK;
 }
 
@@ -921,6 +924,7 @@ lsmf_status lsmf_solver_create(const lsmf_solver_options *o, const double *mu, c
     lsmf_mesh_desc md{o->dim, o->degree, fg.mesh->n_cells, fg.mesh->n_nodes, fg.mesh->cell_nodes, fg.mesh->coords,
                       fg.cons.data(), fg.mesh->cell_colour};
     S->fine.initialize(md, o->n_q_1d, o->caching, S->s);
+    S->fine.set_formulation(o->formulation);
     S->fine.set_material(fg.mu.data(), fg.lam.data());
     if (o->preconditioner != LSMF_PRECONDITIONER_GMG)
       ;  // no level operators

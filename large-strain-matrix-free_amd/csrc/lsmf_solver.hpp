@@ -88,6 +88,7 @@ template <typename number> class NeoHookOperatorHIP {
     abi_check(lsmf_op_create(&mesh, &o, &op_), "NeoHookOperator::initialize");
     stream_ = s;
   }
+  void set_formulation(int formulation) { abi_check(lsmf_set_formulation(op_, formulation), "set_formulation"); }
   void set_material(const double *mu, const double *lambda) {
     abi_check(lsmf_set_material_per_cell(op_, mu, lambda), "set_material");
   }

@@ -68,7 +68,8 @@ class _MeshDesc(ctypes.Structure):
 
 
 class _OpOptions(ctypes.Structure):
-    _fields_ = [("n_q_1d", ctypes.c_int), ("precision", ctypes.c_int), ("caching", ctypes.c_int)]
+    _fields_ = [("n_q_1d", ctypes.c_int), ("precision", ctypes.c_int), ("caching", ctypes.c_int),
+                ("layout", ctypes.c_int)]
 
 
 class _Mesh(ctypes.Structure):
@@ -85,6 +86,7 @@ _sig = {
     "lsmf_op_destroy": (_i, [_vp]),
     "lsmf_set_material_per_cell": (_i, [_vp, _vp, _vp]),
     "lsmf_set_linearization": (_i, [_vp, _vp, _vp]),
+    "lsmf_set_formulation": (_i, [_vp, _i]),
     "lsmf_cache": (_i, [_vp, _vp]),
     "lsmf_compute_diagonal": (_i, [_vp, _vp]),
     "lsmf_vmult": (_i, [_vp, _vp, _vp, _i, ctypes.c_uint, _vp]),
@@ -195,7 +197,7 @@ class NeoHookOperator:
     """Opaque lsmf_op handle with the reference's member functions."""
 
     def __init__(self, mesh, n_q_1d=None, precision="fp64", caching="tensor4", constrained=None,
-                 use_mesh_colouring=True):
+                 use_mesh_colouring=True, formulation=1, layout="auto"):
         self.dim, self.degree = mesh.dim, mesh.degree
         self.n_q_1d = n_q_1d if n_q_1d is not None else mesh.degree + 1
         self.precision = precision
@@ -210,10 +212,13 @@ class NeoHookOperator:
                          None if self._col is None else self._col.ctypes.data)
         if caching not in CACHING:
             raise LsmfError(-2, "Unknown caching")
-        opt = _OpOptions(self.n_q_1d, PRECISION[precision], CACHING[caching])
+        opt = _OpOptions(self.n_q_1d, PRECISION[precision], CACHING[caching], {"auto": 0, "cells": 1}[layout])
         h = _vp()
         _check(_lib.lsmf_op_create(ctypes.byref(desc), ctypes.byref(opt), ctypes.byref(h)))
         self._h = h
+        self.formulation = formulation
+        if formulation != 1:
+            _check(_lib.lsmf_set_formulation(h, formulation))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -350,7 +355,7 @@ class SolverOptions(ctypes.Structure):
                 ("neumann_sides", ctypes.c_uint), ("neumann_traction", (ctypes.c_double * 3) * 6),
                 ("verbose", ctypes.c_int), ("coarse_solver", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("preconditioner", ctypes.c_int), ("preconditioner_relaxation", ctypes.c_double),
-                ("tol_f_abs", ctypes.c_double), ("tol_u_abs", ctypes.c_double)]
+                ("tol_f_abs", ctypes.c_double), ("tol_u_abs", ctypes.c_double), ("formulation", ctypes.c_int)]
 
 
 COARSE = {"chebyshev": 0, "direct": 1}

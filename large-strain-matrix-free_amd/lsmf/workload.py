@@ -110,9 +110,9 @@ CONFIGS = {
     1: dict(dim=3, n=32, degree=2, nu=0.3, n_incl=20, radius=0.08, factor=10.0, caching="tensor4",
             load_steps=5, increment=0.01, amplitude=0.02),
     2: dict(dim=3, n=128, degree=2, nu=0.45, n_incl=200, radius=0.03, factor=10.0, caching="tensor4",
-            load_steps=3, increment=0.03, amplitude=0.02),
+            load_steps=3, increment=0.03, amplitude=0.02, cheb_degree=4),
     3: dict(dim=3, n=48, degree=2, nu=0.3, sigma=0.5, caching="tensor4"),
-    4: dict(dim=3, n=64, degree=4, nu=0.45, sigma=0.3, caching="tensor2"),
+    4: dict(dim=3, n=64, degree=4, nu=0.45, sigma=0.3, caching="tensor2", load_steps=3, increment=0.03),
 }
 
 

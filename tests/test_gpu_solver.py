@@ -214,7 +214,8 @@ def cook_gmg_options(prec, caching):
     return o, mu, 2 * mu * nu / (1 - 2 * nu)
 
 
-@pytest.mark.parametrize("prec,caching", [("fp32", "scalar"), ("fp64", "tensor4"), ("fp32", "tensor2")])
+@pytest.mark.parametrize("prec,caching", [("fp32", "scalar"), ("fp64", "tensor4"), ("fp32", "tensor2"),
+                                         ("fp32", "tensor4_ns"), ("fp32", "scalar_referential")])
 def test_cook_gmg_newton_matches_blessed_output(prec, caching, device):
     log(f"[cook gmg newton] {prec} {caching}")
     out = GOLD["cook_mf_gmg_form_1"]["output"]
@@ -340,3 +341,45 @@ def test_unpreconditioned_and_jacobi_newton_agree_with_gmg(pre, device):
         f" GMG CG {[s['linear_iterations'] for s in st_g]}")
     assert [s["newton_iterations"] for s in st_p] == [s["newton_iterations"] for s in st_g]
     assert np.abs(u_p - u_g).max() < 1e-8 * np.abs(u_g).max()
+
+
+@pytest.mark.parametrize("name,pre,prec,caching", [("cook_mf", "jacobi", "fp64", "scalar"),
+                                                   ("cook_mf_gmg", "gmg", "fp32", "scalar"),
+                                                   ("cook_mf_gmg", "gmg", "fp32", "tensor4"),
+                                                   ("cook_3d_quad_mf", "jacobi", "fp64", "tensor4")])
+def test_cook_formulation0_newton_matches_blessed_output(name, pre, prec, caching, device):
+    """formulation 0 through the solver API: every step's Newton count and
+    point displacement against the reference's blessed output"""
+    log(f"[cook formulation 0] {name} {pre} {prec} {caching}")
+    gold = GOLD[name]
+    prm, out = gold["prm"], gold["output"]
+    dim = int(prm.get("Dimension", "2"))
+    p = int(prm["Polynomial degree"])
+    E, R = int(prm["Elements per edge"]), int(prm.get("Global refinement", "0"))
+    mu = float(prm["Shear modulus"])
+    nu = float(prm["Poisson's ratio"])
+    lam = 2 * mu * nu / (1 - 2 * nu)
+    gmg = pre == "gmg"
+    nc = [E, E] + ([1] if dim == 3 else [])
+    fine = [k << R for k in nc[:2]] + nc[2:]
+    lo, hi = ([0, 0], [48, 44]) if dim == 2 else ([0, 0, -0.5], [48, 44, 0.5])
+    o = lsmf.solver_options(dim=dim, degree=p, n_q_1d=p + 1, n_cells_coarse=nc if gmg else fine,
+                            n_levels=R + 1 if gmg else 1, geometry="cook", lo=lo, hi=hi,
+                            scale=float(prm["Grid scale"]), caching=caching, mg_caching=caching, mg_precision=prec,
+                            preconditioner=pre, formulation=0, tol_lin=float(prm["Residual"]),
+                            max_lin_iterations=out["n_dofs"] * int(prm.get("Max iteration multiplier", "1")),
+                            tol_f=float(prm["Tolerance force"]), tol_u=float(prm["Tolerance displacement"]),
+                            max_newton=int(prm["Max iterations Newton-Raphson"]), dt=float(prm["Time step size"]),
+                            t_end=float(prm["End time"]), dirichlet={0: (list(range(dim)), [0.0] * dim)},
+                            neumann={1: [0.0, 1.0 / 16e-6] + [0.0] * (dim - 2)})
+    mesh = lsmf.Mesh(dim, p, fine, lo=lo, hi=hi, geometry="cook", scale=1e-3)
+    S = lsmf.Solver(o, np.full(mesh.n_cells, mu), np.full(mesh.n_cells, lam))
+    assert S.n_dofs() == out["n_dofs"]
+    node = int(np.argmin(np.sum((mesh.coords - np.array(out["steps"][0]["point"])) ** 2, axis=1)))
+    for k, ref in enumerate(out["steps"]):
+        t, st = S.step()
+        assert st["newton_iterations"] == ref["newton"], k
+        u = S.solution()
+        got = [float(f"{v:.6g}") for v in u[dim * node:dim * node + dim]]
+        assert got == ref["displacement"], (k, got, ref["displacement"])
+    assert S.step() is None

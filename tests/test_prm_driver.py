@@ -12,8 +12,8 @@
     block of src/main.cc
   * seeded material fields written by the driver equal the Python workload
     generator's (same SplitMix64 stream)
-  * cases outside this build (Holes geometry, Formulation 0, assembled
-    solvers) fail with a clear message
+  * cases outside this build (Holes geometry, direct and AMG solvers)
+    fail with a clear message
 No GPU is used: --dry-run stops before any device call.
 """
 import json
@@ -47,7 +47,8 @@ def header(output_text):
     return output_text.split("\nTimestep")[0].rstrip("\n") + "\n"
 
 
-@pytest.mark.parametrize("name", ["cook_mf_form_1", "cook_mf_gmg_form_1", "cook_mat_form_1"])
+@pytest.mark.parametrize("name", ["cook_mf_form_1", "cook_mf_gmg_form_1", "cook_mat_form_1", "cook_mf_form_1_tensor4_ns",
+                                  "cook_mf", "cook_mf_gmg", "cook", "cook_3d_lin_mf", "cook_3d_quad_mf"])
 def test_reference_prm_header_matches_blessed_output(name):
     out, js = describe(os.path.join(PRM, name + ".prm"))
     blessed = open(os.path.join(PRM, name + ".output")).read()
@@ -64,6 +65,7 @@ def test_cook_gmg_setup():
     assert js["max_lin_iterations"] == 2178  # multiplier 1 x n_dofs
     assert js["increment_nonzero"] == 0
     assert js["caching"] == 0  # scalar: the reference's default "MF caching"
+    assert js["formulation"] == 1
 
 
 def test_cook_jacobi_setup():
@@ -202,13 +204,14 @@ def test_parameter_errors(tmp_path, text, msg):
     assert "Exception on processing" in err and "Aborting!" in err
 
 
-@pytest.mark.parametrize("name,msg", [
-    ("cook_mf", "Formulation 0 is not supported"),
-    ("cook_3d_quad_mf", "Formulation 0 is not supported"),
-])
-def test_unsupported_reference_cases_fail_clearly(name, msg):
-    rc, out, err = run(["--dry-run", os.path.join(PRM, name + ".prm")])
-    assert rc == 1 and msg in err
+def test_cook_3d_setup():
+    """3D Cook: one element through the thickness; boundary id 2 (the
+    reference tests |x| = 0.5 for it) selects no face at 12 elements per edge"""
+    _, js = describe(os.path.join(PRM, "cook_3d_quad_mf.prm"))
+    assert js["n_cells_fine"] == [12, 12, 1] and js["n_levels"] == 1 and js["degree"] == 2
+    assert js["dirichlet_components"] == [7, 0, 0, 0, 0, 0]
+    assert js["neumann_sides"] == 2 and js["traction"][1] == [0.0, 0.1 / 16.0e-6, 0.0]
+    assert js["max_lin_iterations"] == 10 * 5625
 
 
 def test_unsupported_settings(tmp_path):
