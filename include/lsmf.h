@@ -97,6 +97,22 @@ typedef struct {
                                  DoF of this operator, or NULL */
   const int32_t *cell_colour; /* host, [n_cells] colouring with no node shared
                                  inside a colour, or NULL (greedy colouring) */
+  /* Hanging-node (general affine, homogeneous) constraints, the constraint
+   * pools deal.II's read_dof_values / distribute_local_to_global resolve per
+   * cell (include/mf_nh_operator.h:745-749, 769-780): node nc_nodes[k] has,
+   * in every component, the value sum_j nc_weights[j] * value(nc_masters[j])
+   * over j in [nc_ptr[k], nc_ptr[k+1]).  Masters must be unconstrained by
+   * this list (closed constraints); a master's Dirichlet components read as
+   * zero.  Constrained nodes' rows act as identity, like Dirichlet rows.
+   * 0 / NULL: none.  With constraints the cells layout is used, the colouring
+   * covers the masters a cell writes, and compute_diagonal probes the
+   * operator with vmults (one per colour of a distance-2 node colouring and
+   * component) instead of the closed form. */
+  int n_node_constraints;
+  const int32_t *nc_nodes;   /* [n_node_constraints] */
+  const int32_t *nc_ptr;     /* [n_node_constraints + 1] */
+  const int32_t *nc_masters; /* [nc_ptr[n]] */
+  const double *nc_weights;  /* [nc_ptr[n]] */
 } lsmf_mesh_desc;
 
 typedef struct {

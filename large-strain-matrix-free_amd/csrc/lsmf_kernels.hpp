@@ -618,6 +618,11 @@ template <typename Number> struct VmultParams {
   int formulation;  // 1, or 0: the volumetric/isochoric split (scalar, tensor4, residual)
   const uint32_t *__restrict__ tnodes;  // tiled layout: [tile][TN] node codes
   const int32_t *__restrict__ tinfo;    // tiled layout: [tile][2] first cell, extents
+  // node (hanging) constraints, cells layout only; hang == nullptr: none
+  const int32_t *__restrict__ hang;     // [sorted cell][NQP] constraint index or -1
+  const int32_t *__restrict__ hptr;     // [n + 1]
+  const uint32_t *__restrict__ hmaster; // master node | its Dirichlet bits
+  const Number *__restrict__ hw;        // weights
   // adjoint: per-cell outputs in mesh order (perm: sorted -> mesh cell)
   double *__restrict__ dmu;
   double *__restrict__ dlambda;
@@ -828,12 +833,23 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
 #pragma unroll
         for (int c = 0; c < DIM; ++c) A[c * NQP + r] = tile_v[ln * DIM + c];
     } else if (active) {
+      const int hidx = P.hang ? P.hang[cell_s * NQP + r_s] : -1;
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
         T v;
-        if constexpr (KIN) v = P.src[node * DIM + c];
-        else v = sv[c];
-        if (CONSTR && ((code >> (kConsShift + c)) & 1u)) v = T(0);
+        if (hidx >= 0) {
+          // a constrained node reads the weighted sum of its masters
+          v = T(0);
+          for (int j = P.hptr[hidx]; j < P.hptr[hidx + 1]; ++j) {
+            const uint32_t mc = P.hmaster[j];
+            if (CONSTR && ((mc >> (kConsShift + c)) & 1u)) continue;
+            v = fma(P.hw[j], P.src[(size_t)(mc & kNodeMask) * DIM + c], v);
+          }
+        } else {
+          if constexpr (KIN) v = P.src[node * DIM + c];
+          else v = sv[c];
+          if (CONSTR && ((code >> (kConsShift + c)) & 1u)) v = T(0);
+        }
         A[c * NQP + r] = v;
       }
     }
@@ -1036,7 +1052,8 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   }
   // colour-ordered scatter (distribute_local_to_global, :769-780): all reads
   // of the node's values are issued before the writes (one round trip)
-  if (active) {
+  const int hidx = (P.hang && active) ? P.hang[cell_s * NQP + r_s] : -1;
+  if (active && hidx < 0) {
     const bool first = !ADD && (code & kFirstBit);
     T old[DIM];
 #pragma unroll
@@ -1050,6 +1067,53 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
       P.dst[node * DIM + c] = old[c] + out[c];
     }
   }
+  if (P.hang) {
+    // constrained nodes' results go to their masters.  Several lanes of one
+    // cell can share a master, so one lane per cell applies them in lane
+    // order, after the cell's own writes (the colouring covers the masters
+    // a cell writes, so no other cell of this colour touches them).
+    lds_barrier();  // every lane is done with the sum-factorization buffers
+    T *hres = lds;  // [lane][DIM] results
+    if (on)
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) hres[t * DIM + c] = out[c];
+    __syncthreads();  // LDS results and the cell's own global writes are visible
+    if (active && r == 0) {
+      for (int rr = 0; rr < NQP; ++rr) {
+        const int k = P.hang[cell_s * NQP + rr];
+        if (k < 0) continue;
+        for (int j = P.hptr[k]; j < P.hptr[k + 1]; ++j) {
+          const uint32_t mc = P.hmaster[j];
+          const size_t m = mc & kNodeMask;
+#pragma unroll
+          for (int c = 0; c < DIM; ++c) {
+            if (CONSTR && ((mc >> (kConsShift + c)) & 1u)) continue;
+            atomicAdd(P.dst + m * DIM + c, P.hw[j] * hres[(slot * NQP + rr) * DIM + c]);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <typename T> __global__ void k_zero(T *__restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = T(0);
+}
+
+// diagonal by probing (meshes with node constraints): x = indicator of the
+// free DoFs (node colour k, component c), and diag = (A x) on them
+template <typename T>
+__global__ void k_probe_set(T *__restrict__ x, const int32_t *__restrict__ node_colour,
+                            const uint8_t *__restrict__ fixed, int k, int c, int dim, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = (node_colour[i / dim] == k && (int)(i % dim) == c && !fixed[i]) ? T(1) : T(0);
+}
+template <typename T>
+__global__ void k_probe_take(T *__restrict__ diag, const T *__restrict__ y, const int32_t *__restrict__ node_colour,
+                             const uint8_t *__restrict__ fixed, int k, int c, int dim, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && node_colour[i / dim] == k && (int)(i % dim) == c && !fixed[i]) diag[i] = y[i];
 }
 
 // constrained rows act as identity (include/mf_nh_operator.h:703-705)

@@ -27,6 +27,10 @@ template <int DIM, int N> struct Launch {
     P.formulation = op->formulation;
     P.tnodes = op->d_tile_nodes;
     P.tinfo = op->d_tile_info;
+    P.hang = op->d_hang;
+    P.hptr = op->d_hptr;
+    P.hmaster = op->d_hmaster;
+    P.hw = static_cast<const T *>(op->d_hw);
     P.dmu = op->adj_dmu;
     P.dlambda = op->adj_dlambda;
     P.perm = op->d_perm;
@@ -34,6 +38,11 @@ template <int DIM, int N> struct Launch {
   }
 
   static int grid_for(int n) { return (n + CS::CPW - 1) / CS::CPW; }
+  // with node constraints every write adds (no first-touch stores): zero first
+  template <typename T> static void zero_if_constrained(const lsmf_op *op, T *dst, bool add, hipStream_t s) {
+    if (op->n_node_cons > 0 && !add)
+      hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((op->n_dofs + 255) / 256)), dim3(256), 0, s, dst, op->n_dofs);
+  }
   static int n_colours(const lsmf_op *op) {
     return (int)(op->tiled ? op->tile_ptr.size() : op->colour_ptr.size()) - 1;
   }
@@ -64,6 +73,7 @@ template <int DIM, int N> struct Launch {
     T *dst = static_cast<T *>(dst_);
     const T *src = static_cast<const T *>(src_);
     constexpr unsigned FULL = PH_SF | PH_QD;
+    zero_if_constrained(op, dst, add, s);
     if (constr) {
       if (ph == FULL) {
         if (add) run_colours<T, VAR, true, true, FULL>(op, dst, src, s);
@@ -143,6 +153,7 @@ template <int DIM, int N> struct Launch {
   }
 
   template <typename T, int VAR> static int diagonal_t(const lsmf_op *op, hipStream_t s) {
+    if (op->n_node_cons > 0) return diagonal_probe_t<T, VAR>(op, s);
     const int nc = n_colours(op);
     for (int k = 0; k < nc; ++k) {
       VmultParams<T> P = params<T>(op, k);
@@ -160,6 +171,30 @@ template <int DIM, int N> struct Launch {
     const size_t n = op->n_dofs;
     hipLaunchKernelGGL(k_invert<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, diag,
                        static_cast<T *>(op->d_inv_diag), n);
+    return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
+  }
+
+  // node constraints: diag(A) by probing, one vmult per node colour and
+  // component (nodes a common cell writes differ in colour, so each probe
+  // reads its DoFs' diagonal entries without interference)
+  template <typename T, int VAR> static int diagonal_probe_t(const lsmf_op *op, hipStream_t s) {
+    T *diag = static_cast<T *>(op->d_diag);
+    T *x = static_cast<T *>(op->d_inv_diag);  // scratch until inverted below
+    T *y = static_cast<T *>(op->d_probe);
+    const size_t n = op->n_dofs;
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+    constexpr unsigned FULL = PH_SF | PH_QD;
+    hipLaunchKernelGGL(k_zero<T>, g, b, 0, s, diag, n);
+    for (int k = 0; k < op->n_node_colours; ++k)
+      for (int c = 0; c < DIM; ++c) {
+        hipLaunchKernelGGL(k_probe_set<T>, g, b, 0, s, x, op->d_node_colour, op->d_fixed, k, c, DIM, n);
+        zero_if_constrained(op, y, false, s);
+        run_colours<T, VAR, false, true, FULL>(op, y, x, s);
+        hipLaunchKernelGGL(k_probe_take<T>, g, b, 0, s, diag, y, op->d_node_colour, op->d_fixed, k, c, DIM, n);
+      }
+    if (op->n_cons > 0)
+      hipLaunchKernelGGL(k_set_one<T>, dim3((op->n_cons + 255) / 256), dim3(256), 0, s, op->d_cons, op->n_cons, diag);
+    hipLaunchKernelGGL(k_invert<T>, g, b, 0, s, diag, static_cast<T *>(op->d_inv_diag), n);
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
 
@@ -183,6 +218,7 @@ template <int DIM, int N> struct Launch {
   static int residual(const lsmf_op *op, double *rhs, hipStream_t s) {
     if (op->precision != LSMF_FP64) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual needs an FP64 operator");
     constexpr unsigned FULL = PH_SF | PH_QD;
+    zero_if_constrained(op, rhs, false, s);
     run_colours<double, V_RESIDUAL, false, false, FULL>(op, rhs, nullptr, s);
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
@@ -190,6 +226,7 @@ template <int DIM, int N> struct Launch {
   static int residual_vjp(const lsmf_op *op, double *du, const double *w, hipStream_t s) {
     if (op->precision != LSMF_FP64) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual VJP needs an FP64 operator");
     constexpr unsigned FULL = PH_SF | PH_QD;
+    zero_if_constrained(op, du, false, s);
     run_colours<double, V_ADJOINT, false, false, FULL>(op, du, w, s);
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }

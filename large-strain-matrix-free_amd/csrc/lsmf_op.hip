@@ -79,7 +79,8 @@ template <typename T> int dalloc(T **p, size_t n, const char *what) {
 void free_op(lsmf_op *op) {
   void *ptrs[] = {op->d_idx,  op->d_coords,    op->d_tab,  op->d_tab64, op->d_mu64,     op->d_lambda64,
                   op->d_mu,   op->d_lambda,    op->d_cache, op->d_diag, op->d_inv_diag, op->d_cons,
-                  op->d_bad,  op->d_perm,      op->d_tile_nodes, op->d_tile_info};
+                  op->d_bad,  op->d_perm,      op->d_tile_nodes, op->d_tile_info, op->d_hang, op->d_hptr,
+                  op->d_hmaster, op->d_hw, op->d_node_colour, op->d_fixed, op->d_probe};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (op->d_u64) (void)hipFree(op->d_u64);
@@ -269,6 +270,54 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   op->n_fields = n_cache_fields(dim, opt->caching);
   op->field_stride = (size_t)nc * nqp;
 
+  // --- node (hanging) constraints: validated, closed (masters are free)
+  const int nnc = mesh->n_node_constraints;
+  std::vector<int> hang_of;
+  if (nnc < 0 || (nnc > 0 && (!mesh->nc_nodes || !mesh->nc_ptr || !mesh->nc_masters || !mesh->nc_weights))) {
+    delete op;
+    return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: bad node constraint arrays");
+  }
+  if (nnc > 0) {
+    hang_of.assign(mesh->n_nodes, -1);
+    if (mesh->nc_ptr[0] != 0) {
+      delete op;
+      return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: nc_ptr[0] must be 0");
+    }
+    for (int k = 0; k < nnc; ++k) {
+      const int n = mesh->nc_nodes[k];
+      if (n < 0 || n >= mesh->n_nodes || hang_of[n] >= 0 || mesh->nc_ptr[k + 1] < mesh->nc_ptr[k]) {
+        delete op;
+        return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: bad constrained node or nc_ptr");
+      }
+      hang_of[n] = k;
+      if (mesh->constrained)
+        for (int c = 0; c < dim; ++c)
+          if (mesh->constrained[(size_t)n * dim + c]) {
+            delete op;
+            return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: a node is both constrained and Dirichlet");
+          }
+    }
+    for (int j = 0; j < mesh->nc_ptr[nnc]; ++j) {
+      const int m = mesh->nc_masters[j];
+      if (m < 0 || m >= mesh->n_nodes || hang_of[m] >= 0) {
+        delete op;
+        return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: masters must be unconstrained nodes");
+      }
+    }
+  }
+  op->n_node_cons = nnc;
+  // nodes cell c writes: its own nodes and the masters of its constrained ones
+  auto write_nodes = [&](int c, std::vector<int> &w) {
+    w.clear();
+    for (int a = 0; a < nqp; ++a) {
+      const int n = mesh->cell_nodes[(size_t)c * nqp + a];
+      w.push_back(n);
+      if (nnc > 0 && hang_of[n] >= 0)
+        for (int j = mesh->nc_ptr[hang_of[n]]; j < mesh->nc_ptr[hang_of[n] + 1]; ++j) w.push_back(mesh->nc_masters[j]);
+    }
+  };
+  std::vector<int> wbuf;
+
   // --- colouring (given, checked below, or greedy first fit)
   std::vector<int> colour(nc, 0);
   if (mesh->cell_colour) {
@@ -282,8 +331,9 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   } else {
     std::vector<uint64_t> used(mesh->n_nodes, 0);
     for (int c = 0; c < nc; ++c) {
+      write_nodes(c, wbuf);
       uint64_t m = 0;
-      for (int a = 0; a < nqp; ++a) m |= used[mesh->cell_nodes[(size_t)c * nqp + a]];
+      for (int n : wbuf) m |= used[n];
       int col = 0;
       while (col < 64 && (m >> col) & 1ull) ++col;
       if (col == 64) {
@@ -291,7 +341,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
         return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: greedy colouring needs more than 64 colours");
       }
       colour[c] = col;
-      for (int a = 0; a < nqp; ++a) used[mesh->cell_nodes[(size_t)c * nqp + a]] |= (1ull << col);
+      for (int n : wbuf) used[n] |= (1ull << col);
     }
   }
   std::vector<int> min_col;
@@ -310,19 +360,21 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       op->inv_perm[c] = pos;
     }
   }
-  // colour conflict check: no node twice inside one colour; min colour per node
-  std::vector<int> stamp(mesh->n_nodes, -1);
+  // colour conflict check: no node written by two cells of one colour; min
+  // colour per node
+  std::vector<int> stamp(mesh->n_nodes, -1), stamp_cell(mesh->n_nodes, -1);
   min_col.assign(mesh->n_nodes, 1 << 30);
   for (int k = 0; k < n_col; ++k)
     for (int pos = op->colour_ptr[k]; pos < op->colour_ptr[k + 1]; ++pos) {
       const int c = op->perm[pos];
-      for (int a = 0; a < nqp; ++a) {
-        const int nd = mesh->cell_nodes[(size_t)c * nqp + a];
-        if (stamp[nd] == k) {
+      write_nodes(c, wbuf);
+      for (int nd : wbuf) {
+        if (stamp[nd] == k && stamp_cell[nd] != c) {
           delete op;
           return fail(LSMF_ERR_INVALID_ARGUMENT, "lsmf_op_create: two cells of one colour share a node");
         }
         stamp[nd] = k;
+        stamp_cell[nd] = c;
         min_col[nd] = std::min(min_col[nd], k);
       }
     }
@@ -331,7 +383,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   std::vector<uint32_t> tnodes;
   std::vector<int32_t> tinfo;
   long long sn[3] = {1, 1, 1};
-  op->tiled = opt->layout == LSMF_LAYOUT_AUTO && structured_dims(mesh, dim, p, nqp, sn);
+  op->tiled = opt->layout == LSMF_LAYOUT_AUTO && nnc == 0 && structured_dims(mesh, dim, p, nqp, sn);
   if (op->tiled) {
     const int rc = build_tiles(op, mesh, sn, tnodes, tinfo);
     if (rc) {
@@ -349,7 +401,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       if (mesh->constrained)
         for (int comp = 0; comp < dim; ++comp)
           if (mesh->constrained[(size_t)nd * dim + comp]) code |= 1u << (kConsShift + comp);
-      if (!op->tiled && colour[c] == min_col[nd]) code |= kFirstBit;
+      if (!op->tiled && nnc == 0 && colour[c] == min_col[nd]) code |= kFirstBit;
       idx[(size_t)pos * nqp + a] = code;
     }
   }
@@ -357,7 +409,58 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   if (mesh->constrained)
     for (size_t i = 0; i < op->n_dofs; ++i)
       if (mesh->constrained[i]) cons.push_back((int32_t)i);
+  // constrained nodes' rows act as identity, like Dirichlet rows
+  for (int k = 0; k < nnc; ++k)
+    for (int c = 0; c < dim; ++c) cons.push_back(mesh->nc_nodes[k] * dim + c);
   op->n_cons = (int)cons.size();
+
+  // --- node-constraint tables and the distance-2 node colouring that the
+  // probing diagonal uses (nodes written by a common cell differ in colour)
+  std::vector<int32_t> hang_tab, hptr, ncolour;
+  std::vector<uint32_t> hmaster;
+  std::vector<double> hw;
+  std::vector<uint8_t> fixed;
+  if (nnc > 0) {
+    hang_tab.resize((size_t)nc * nqp);
+    for (int pos = 0; pos < nc; ++pos)
+      for (int a = 0; a < nqp; ++a)
+        hang_tab[(size_t)pos * nqp + a] = hang_of[mesh->cell_nodes[(size_t)op->perm[pos] * nqp + a]];
+    hptr.assign(mesh->nc_ptr, mesh->nc_ptr + nnc + 1);
+    for (int j = 0; j < mesh->nc_ptr[nnc]; ++j) {
+      const int m = mesh->nc_masters[j];
+      uint32_t code = (uint32_t)m;
+      if (mesh->constrained)
+        for (int c = 0; c < dim; ++c)
+          if (mesh->constrained[(size_t)m * dim + c]) code |= 1u << (kConsShift + c);
+      hmaster.push_back(code);
+      hw.push_back(mesh->nc_weights[j]);
+    }
+    fixed.assign(op->n_dofs, 0);
+    for (int32_t d : cons) fixed[d] = 1;
+    std::vector<std::vector<int>> node_cells(mesh->n_nodes);
+    for (int c = 0; c < nc; ++c) {
+      write_nodes(c, wbuf);
+      for (int n : wbuf) node_cells[n].push_back(c);
+    }
+    ncolour.assign(mesh->n_nodes, -1);
+    std::vector<int> mark;
+    int n_colours = 0;
+    for (int n = 0; n < mesh->n_nodes; ++n) {
+      for (int c : node_cells[n]) {
+        write_nodes(c, wbuf);
+        for (int m : wbuf)
+          if (ncolour[m] >= 0) {
+            if ((int)mark.size() <= ncolour[m]) mark.resize(ncolour[m] + 1, -1);
+            mark[ncolour[m]] = n;
+          }
+      }
+      int col = 0;
+      while (col < (int)mark.size() && mark[col] == n) ++col;
+      ncolour[n] = col;
+      n_colours = std::max(n_colours, col + 1);
+    }
+    op->n_node_colours = n_colours;
+  }
 
   // --- 1D tables: S, D, St, Dt, qp, qw, and the collocation derivative on
   // the Gauss points Dc[q][k] = l_k'(x_q) (l_k: Lagrange basis on the Gauss
@@ -395,7 +498,14 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       (rc = dalloc(&op->d_cons, cons.size(), "constrained list")) || (rc = dalloc(&op->d_bad, 1, "counter")) ||
       (rc = dalloc(&op->d_perm, (size_t)nc, "cell permutation")) ||
       (rc = dalloc(&op->d_tile_nodes, tnodes.size(), "tile nodes")) ||
-      (rc = dalloc(&op->d_tile_info, tinfo.size(), "tile info"))) {
+      (rc = dalloc(&op->d_tile_info, tinfo.size(), "tile info")) ||
+      (nnc > 0 && ((rc = dalloc(&op->d_hang, hang_tab.size(), "constraint table")) ||
+                   (rc = dalloc(&op->d_hptr, hptr.size(), "constraint offsets")) ||
+                   (rc = dalloc(&op->d_hmaster, hmaster.size(), "constraint masters")) ||
+                   (rc = dalloc((char **)&op->d_hw, hw.size() * es, "constraint weights")) ||
+                   (rc = dalloc(&op->d_node_colour, ncolour.size(), "node colours")) ||
+                   (rc = dalloc(&op->d_fixed, fixed.size(), "fixed DoFs")) ||
+                   (rc = dalloc((char **)&op->d_probe, op->n_dofs * es, "probe vector"))))) {
     free_op(op);
     delete op;
     return rc;
@@ -428,6 +538,23 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
     e = hipMemcpy(op->d_tile_nodes, tnodes.data(), tnodes.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess && !tinfo.empty())
     e = hipMemcpy(op->d_tile_info, tinfo.data(), tinfo.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnc > 0) {
+    e = hipMemcpy(op->d_hang, hang_tab.data(), hang_tab.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(op->d_hptr, hptr.data(), hptr.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(op->d_hmaster, hmaster.data(), hmaster.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      if (op->precision == LSMF_FP64) {
+        e = hipMemcpy(op->d_hw, hw.data(), hw.size() * sizeof(double), hipMemcpyHostToDevice);
+      } else {
+        std::vector<float> hf(hw.begin(), hw.end());
+        e = hipMemcpy(op->d_hw, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice);
+      }
+    }
+    if (e == hipSuccess)
+      e = hipMemcpy(op->d_node_colour, ncolour.data(), ncolour.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(op->d_fixed, fixed.data(), fixed.size(), hipMemcpyHostToDevice);
+  }
   op->bytes_idx += tnodes.size() * sizeof(uint32_t) + tinfo.size() * sizeof(int32_t);
   if (e != hipSuccess) {
     free_op(op);

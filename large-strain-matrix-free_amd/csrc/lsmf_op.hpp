@@ -80,6 +80,16 @@ struct lsmf_op {
   std::vector<int> tile_ptr;
   uint32_t *d_tile_nodes = nullptr;  // [n_tiles][TN] node | constraint bits | first-touch bit, or kNoNode
   int32_t *d_tile_info = nullptr;    // [n_tiles][2] first sorted cell, extents ex | ey << 8 | ez << 16
+  // node (hanging) constraints: cells layout, all-add scatter after a zero
+  // pass, diagonal by probing over a distance-2 node colouring
+  int n_node_cons = 0;
+  int32_t *d_hang = nullptr, *d_hptr = nullptr;
+  uint32_t *d_hmaster = nullptr;
+  void *d_hw = nullptr;
+  int32_t *d_node_colour = nullptr;
+  int n_node_colours = 0;
+  uint8_t *d_fixed = nullptr;  // per DoF: Dirichlet or constrained node
+  void *d_probe = nullptr;     // probing diagonal: A x
   bool material_set = false, linearization_set = false, cached = false, diag_ok = false;
   size_t bytes_idx = 0, bytes_cache = 0;
 };

@@ -64,7 +64,9 @@ class LsmfError(RuntimeError):
 class _MeshDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("degree", ctypes.c_int), ("n_cells", ctypes.c_int),
                 ("n_nodes", ctypes.c_int), ("cell_nodes", ctypes.c_void_p), ("coords", ctypes.c_void_p),
-                ("constrained", ctypes.c_void_p), ("cell_colour", ctypes.c_void_p)]
+                ("constrained", ctypes.c_void_p), ("cell_colour", ctypes.c_void_p),
+                ("n_node_constraints", ctypes.c_int), ("nc_nodes", ctypes.c_void_p), ("nc_ptr", ctypes.c_void_p),
+                ("nc_masters", ctypes.c_void_p), ("nc_weights", ctypes.c_void_p)]
 
 
 class _OpOptions(ctypes.Structure):
@@ -197,7 +199,7 @@ class NeoHookOperator:
     """Opaque lsmf_op handle with the reference's member functions."""
 
     def __init__(self, mesh, n_q_1d=None, precision="fp64", caching="tensor4", constrained=None,
-                 use_mesh_colouring=True, formulation=1, layout="auto"):
+                 use_mesh_colouring=True, formulation=1, layout="auto", node_constraints=None):
         self.dim, self.degree = mesh.dim, mesh.degree
         self.n_q_1d = n_q_1d if n_q_1d is not None else mesh.degree + 1
         self.precision = precision
@@ -210,6 +212,13 @@ class NeoHookOperator:
         desc = _MeshDesc(self.dim, self.degree, self.n_cells, self.n_nodes, self._cn.ctypes.data,
                          self._xyz.ctypes.data, None if self._cons is None else self._cons.ctypes.data,
                          None if self._col is None else self._col.ctypes.data)
+        if node_constraints is not None:
+            # (nodes, ptr, masters, weights): hanging-node constraints
+            nodes, ptr, masters, weights = node_constraints
+            self._nc = [np.ascontiguousarray(nodes, np.int32), np.ascontiguousarray(ptr, np.int32),
+                        np.ascontiguousarray(masters, np.int32), np.ascontiguousarray(weights, np.float64)]
+            desc.n_node_constraints = len(self._nc[0])
+            desc.nc_nodes, desc.nc_ptr, desc.nc_masters, desc.nc_weights = [a.ctypes.data for a in self._nc]
         if caching not in CACHING:
             raise LsmfError(-2, "Unknown caching")
         opt = _OpOptions(self.n_q_1d, PRECISION[precision], CACHING[caching], {"auto": 0, "cells": 1}[layout])

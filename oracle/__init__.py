@@ -51,6 +51,7 @@ def lib():
             "orc_op_set_linearization": (i, [vp, vp]),
             "orc_op_cache": (i, [vp]),
             "orc_op_set_formulation": (i, [vp, i]),
+            "orc_op_set_node_constraints": (i, [vp, i, vp, vp, vp, vp]),
             "orc_op_vmult": (i, [vp, vp, vp, i, i]),
             "orc_op_diagonal": (i, [vp, vp]),
             "orc_op_residual": (i, [vp, vp]),
@@ -114,6 +115,14 @@ class OracleOperator:
         if getattr(self, "_h", None):
             self._destroy(self._h)
             self._h = None
+
+    def set_node_constraints(self, nodes, ptr, masters, weights):
+        """node nodes[k] = sum_j weights[j] * masters[j], j in [ptr[k], ptr[k+1])"""
+        self._nc = [np.ascontiguousarray(nodes, np.int32), np.ascontiguousarray(ptr, np.int32),
+                    np.ascontiguousarray(masters, np.int32), np.ascontiguousarray(weights, np.float64)]
+        rc = lib().orc_op_set_node_constraints(self._h, len(self._nc[0]), *[_p(a) for a in self._nc])
+        if rc != 0:
+            raise ValueError("oracle: masters must be unconstrained nodes")
 
     def set_material(self, mu, lam):
         self._mu = np.ascontiguousarray(mu, dtype=np.float64)

@@ -19,6 +19,9 @@
 //   * residual in assemble_system() .......... include/mf_elasticity.h:2758-2804,
 //                                              2859-2898, 2917-2921
 //   * make_constraints (increment at it 0) ... include/mf_elasticity.h:2932-3007
+//   * hanging-node (general affine) constraints in read_dof_values /
+//     distribute_local_to_global, as deal.II's AffineConstraints resolve
+//     them (include/mf_nh_operator.h:745-749, 769-780)
 //   * solve_linear_system (CG, Jacobi) ....... include/mf_elasticity.h:3012-3214
 //   * check_convergence / Newton loop ........ include/mf_elasticity.h:2245-2559
 //   * Time stepping (t accumulates dt) ....... include/mf_elasticity.h:614-653,
@@ -190,6 +193,8 @@ struct OpBase {
   virtual double volume() = 0;
   virtual int n_colours() = 0;
   virtual int min_detF(double *out) = 0;
+  virtual int set_node_constraints(int n, const int *nodes, const int *ptr, const int *masters,
+                                   const double *weights) = 0;
   int dim = 0, p = 0, nq = 0, n_cells = 0, n_nodes = 0, caching = 0;
   // Material_Compressible_Neo_Hook_One_Field's formulation (include/material.h:82-94):
   // 1 = mu/2 (tr C - d - 2 ln J) + lambda ln^2 J; 0 = volumetric/isochoric split
@@ -217,6 +222,10 @@ template <int DIM, int P, int NQ> struct Op : OpBase {
   std::vector<int> colour_of, colour_ptr, colour_cells;
   // per qp metric
   std::vector<double> jinv_ref, jxw_ref, jinv_cur, jxw_cur;
+  // node constraints: node n's values = sum_j hc_w[j] values(hc_m[j]),
+  // j in [hc_ptr[k], hc_ptr[k+1]) with k = hang_of[n] (-1: free)
+  std::vector<int> hang_of, hc_ptr, hc_m;
+  std::vector<double> hc_w;
   // caches (restating the reference's Table<2,...> members)
   std::vector<double> c_s, c_s2, c_t2, c_t4, c_t4ns, c_uqp;
   bool cached = false;
@@ -254,20 +263,35 @@ template <int DIM, int P, int NQ> struct Op : OpBase {
     reference_metric();
   }
 
-  // Greedy first-fit colouring: no two cells of one colour share a node, so a
-  // colour's scatter-adds can run in parallel without write conflicts.
+  // nodes a cell writes: its own nodes and the masters of its constrained ones
+  std::vector<int> write_set(int c) const {
+    std::vector<int> w;
+    for (int a = 0; a < NDOF1; ++a) {
+      const int n = cell_nodes[(size_t)c * NDOF1 + a];
+      w.push_back(n);
+      if (!hang_of.empty() && hang_of[n] >= 0)
+        for (int j = hc_ptr[hang_of[n]]; j < hc_ptr[hang_of[n] + 1]; ++j) w.push_back(hc_m[j]);
+    }
+    std::sort(w.begin(), w.end());
+    w.erase(std::unique(w.begin(), w.end()), w.end());
+    return w;
+  }
+
+  // Greedy first-fit colouring: no two cells of one colour write a common
+  // node, so a colour's scatter-adds can run in parallel without conflicts.
   void colour() {
     std::vector<uint64_t> used(n_nodes, 0);
     colour_of.assign(n_cells, 0);
     int nc = 0;
     for (int c = 0; c < n_cells; ++c) {
+      const std::vector<int> ws = write_set(c);
       uint64_t m = 0;
-      for (int a = 0; a < NDOF1; ++a) m |= used[cell_nodes[(size_t)c * NDOF1 + a]];
+      for (int n : ws) m |= used[n];
       int col = 0;
       while (m & (1ull << col)) ++col;
       colour_of[c] = col;
       nc = std::max(nc, col + 1);
-      for (int a = 0; a < NDOF1; ++a) used[cell_nodes[(size_t)c * NDOF1 + a]] |= (1ull << col);
+      for (int n : ws) used[n] |= (1ull << col);
     }
     colour_ptr.assign(nc + 1, 0);
     for (int c = 0; c < n_cells; ++c) colour_ptr[colour_of[c] + 1]++;
@@ -493,24 +517,64 @@ template <int DIM, int P, int NQ> struct Op : OpBase {
       }
   }
 
+  bool hanging(int n) const { return !hang_of.empty() && hang_of[n] >= 0; }
+
+  // read_dof_values: Dirichlet DoFs read as zero (apply_constraints),
+  // constrained nodes as the weighted sum of their masters
   void gather(const double *vec, int cell, double *loc, bool apply_constraints) const {
     for (int a = 0; a < NDOF1; ++a) {
       const int n = cell_nodes[(size_t)cell * NDOF1 + a];
       for (int c = 0; c < DIM; ++c) {
         const size_t dof = (size_t)n * DIM + c;
-        loc[c * NDOF1 + a] = (apply_constraints && cons[dof]) ? 0.0 : vec[dof];
+        double v;
+        if (hanging(n)) {
+          // node constraints belong to the space: applied also without the
+          // Dirichlet ones (the plain product of the Dirichlet lifting)
+          v = 0.0;
+          for (int j = hc_ptr[hang_of[n]]; j < hc_ptr[hang_of[n] + 1]; ++j) {
+            const size_t md = (size_t)hc_m[j] * DIM + c;
+            if (!(apply_constraints && cons[md])) v += hc_w[j] * vec[md];
+          }
+        } else {
+          v = (apply_constraints && cons[dof]) ? 0.0 : vec[dof];
+        }
+        loc[c * NDOF1 + a] = v;
       }
     }
   }
-  void scatter(double *vec, int cell, const double *loc, bool apply_constraints) const {
+  // distribute_local_to_global: Dirichlet rows skipped (apply_constraints),
+  // constrained nodes' entries distributed to their masters (hanging)
+  void scatter(double *vec, int cell, const double *loc, bool apply_constraints, bool hang = true) const {
     for (int a = 0; a < NDOF1; ++a) {
       const int n = cell_nodes[(size_t)cell * NDOF1 + a];
       for (int c = 0; c < DIM; ++c) {
         const size_t dof = (size_t)n * DIM + c;
+        if (hang && hanging(n)) {
+          for (int j = hc_ptr[hang_of[n]]; j < hc_ptr[hang_of[n] + 1]; ++j) {
+            const size_t md = (size_t)hc_m[j] * DIM + c;
+            if (apply_constraints && cons[md]) continue;
+            vec[md] += hc_w[j] * loc[c * NDOF1 + a];
+          }
+          continue;
+        }
         if (apply_constraints && cons[dof]) continue;
         vec[dof] += loc[c * NDOF1 + a];
       }
     }
+  }
+
+  int set_node_constraints(int n, const int *nodes, const int *ptr, const int *masters,
+                           const double *weights) override {
+    hang_of.assign(n_nodes, -1);
+    hc_ptr.assign(ptr, ptr + n + 1);
+    hc_m.assign(masters, masters + ptr[n]);
+    hc_w.assign(weights, weights + ptr[n]);
+    for (int k = 0; k < n; ++k) hang_of[nodes[k]] = k;
+    for (int j = 0; j < ptr[n]; ++j)
+      if (hang_of[masters[j]] >= 0) return -1;  // masters must be free nodes
+    colour();
+    cached = false;
+    return 0;
   }
 
   int set_material(const double *mu_, const double *lambda_) override {
@@ -826,7 +890,7 @@ template <int DIM, int P, int NQ> struct Op : OpBase {
     }
     if (ac)
       for (size_t i = 0; i < nd; ++i)
-        if (cons[i]) dst[i] += src[i];
+        if (cons[i] || hanging((int)(i / DIM))) dst[i] += src[i];
     return 0;
   }
 
@@ -847,11 +911,36 @@ template <int DIM, int P, int NQ> struct Op : OpBase {
           cell_op(cell, in, out);
           dl[i] = out[i];
         }
-        scatter(diag, cell, dl, true);
+        if (hang_of.empty()) {
+          scatter(diag, cell, dl, true);
+          continue;
+        }
+        // with node constraints probe the constrained space: for every DoF
+        // (m, c) the cell writes, in = C_cell e_(m,c), diag += in . A_cell in
+        for (int m : write_set(cell)) {
+          if (hanging(m)) continue;
+          for (int c = 0; c < DIM; ++c) {
+            if (cons[(size_t)m * DIM + c]) continue;
+            std::fill(in, in + DIM * NDOF1, 0.0);
+            bool any = false;
+            for (int a = 0; a < NDOF1; ++a) {
+              const int n = cell_nodes[(size_t)cell * NDOF1 + a];
+              if (n == m) in[c * NDOF1 + a] = 1.0, any = true;
+              if (hanging(n))
+                for (int j = hc_ptr[hang_of[n]]; j < hc_ptr[hang_of[n] + 1]; ++j)
+                  if (hc_m[j] == m) in[c * NDOF1 + a] += hc_w[j], any = true;
+            }
+            if (!any) continue;
+            cell_op(cell, in, out);
+            double v = 0;
+            for (int i = 0; i < DIM * NDOF1; ++i) v += in[i] * out[i];
+            diag[(size_t)m * DIM + c] += v;
+          }
+        }
       }
     }
     for (size_t i = 0; i < nd; ++i)
-      if (cons[i]) diag[i] = 1.0;
+      if (cons[i] || hanging((int)(i / DIM))) diag[i] = 1.0;
     return 0;
   }
 
@@ -1078,6 +1167,10 @@ int orc_op_set_material(void *op, const double *mu, const double *lambda) {
 }
 int orc_op_set_linearization(void *op, const double *u) { return static_cast<orc::OpBase *>(op)->set_linearization(u); }
 int orc_op_cache(void *op) { return static_cast<orc::OpBase *>(op)->cache(); }
+int orc_op_set_node_constraints(void *op, int n, const int *nodes, const int *ptr, const int *masters,
+                                const double *weights) {
+  return static_cast<orc::OpBase *>(op)->set_node_constraints(n, nodes, ptr, masters, weights);
+}
 int orc_op_set_formulation(void *op, int formulation) {
   if (formulation != 0 && formulation != 1) return -1;
   static_cast<orc::OpBase *>(op)->formulation = formulation;

@@ -118,3 +118,23 @@ def test_workload_generators_are_seeded():
     assert np.allclose(mu[~inc], mu0) and np.allclose(mu[inc], 10 * mu0) and inc.any()
     z = lsmf.SplitMix64(3).normal(200000)
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+
+
+def test_node_constraint_errors_before_device_work():
+    import hanging_mesh
+    mesh, (nodes, ptr, masters, weights) = hanging_mesh.build(2, 2)
+    bad = masters.copy()
+    bad[0] = nodes[1]  # a master that is constrained itself
+    with pytest.raises(lsmf.LsmfError) as e:
+        lsmf.NeoHookOperator(mesh, use_mesh_colouring=False, node_constraints=(nodes, ptr, bad, weights))
+    assert e.value.code == -1 and b"masters must be unconstrained" in lsmf._lib.lsmf_last_error()
+    dup = nodes.copy()
+    dup[1] = dup[0]
+    with pytest.raises(lsmf.LsmfError):
+        lsmf.NeoHookOperator(mesh, use_mesh_colouring=False, node_constraints=(dup, ptr, masters, weights))
+    cons = np.zeros(mesh.n_dofs, np.uint8)
+    cons[nodes[0] * 2] = 1
+    with pytest.raises(lsmf.LsmfError):
+        lsmf.NeoHookOperator(mesh, use_mesh_colouring=False, constrained=cons,
+                             node_constraints=(nodes, ptr, masters, weights))
+    assert b"both constrained and Dirichlet" in lsmf._lib.lsmf_last_error()
