@@ -106,3 +106,18 @@ end
         assert np.allclose(mesh.coords[node], [x, y])
         ref = [float(f"{v:.6g}") for v in u[node]]
         assert np.allclose(g, ref, rtol=2e-5, atol=1e-12), (g, ref)
+
+
+def test_benchmark_vmult_timings(tmp_path, device):
+    """Misc/Benchmark vmult: the reference's per-step vmult timing loop adds
+    the vmult (MF) rows of its timings table"""
+    src = open(os.path.join(PRM, "cook_mf_gmg_form_1.prm")).read()
+    src = src.replace("subsection Time", "subsection Misc\n  set Benchmark vmult = 3\nend\n\nsubsection Time")
+    src = src.replace("set End time       = 1", "set End time       = 0.2")
+    path = tmp_path / "bench.prm"
+    path.write_text(src)
+    out, timings = run_driver(str(path), str(tmp_path))
+    assert out.count("Converged in 4 Newton iterations") == 2
+    for row in ("| vmult (MF) ", "| vmult (MF) Cell loop", "| vmult (MF) RW ", "| vmult (MF) RWSF"):
+        line = [l for l in timings.splitlines() if l.startswith(row)]
+        assert line and line[0].split("|")[2].strip() == "6", (row, line)
