@@ -300,6 +300,10 @@ typedef struct {
   double preconditioner_relaxation; /* Jacobi damping omega ("Preconditioner relaxation") */
   double tol_f_abs, tol_u_abs;   /* "Absolute tolerance force" / "... displacement" (:2262-2266) */
   int formulation;               /* material formulation, "Formulation" (1 or 0) */
+  int cond_number_cg_iterations; /* "Condition number CG iterations": an unpreconditioned CG of
+                                    this many steps before each linear solve estimates the
+                                    operator's condition number from its Lanczos coefficients
+                                    (include/mf_elasticity.h:3029-3046); 0 skips it */
 } lsmf_solver_options;
 
 enum { LSMF_PRECONDITIONER_GMG = 0, LSMF_PRECONDITIONER_JACOBI = 1, LSMF_PRECONDITIONER_NONE = 2 };
@@ -315,6 +319,7 @@ typedef struct {
   int max_linear_iterations;
   double residual_0, residual_final;
   double time_total_s, time_setup_s, time_linear_s, time_residual_s;
+  double cond_number;     /* last estimate of the step (0 when not estimated) */
 } lsmf_step_stats;
 
 typedef struct lsmf_solver lsmf_solver;

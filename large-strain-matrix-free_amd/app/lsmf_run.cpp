@@ -234,6 +234,7 @@ void setup(Problem &pr, int dim, int degree, int n_q) {
                          : (P.preconditioner_type == "jacobi" ? LSMF_PRECONDITIONER_JACOBI : LSMF_PRECONDITIONER_NONE);
   o.preconditioner_relaxation = P.preconditioner_relaxation;
   o.formulation = P.material_formulation;
+  o.cond_number_cg_iterations = P.cond_number_cg_iterations;
   for (const auto &kv : P.dirichlet) {
     auto m = P.dirichlet_mask.find(kv.first);
     if (m == P.dirichlet_mask.end())
@@ -615,6 +616,7 @@ int run(const std::string &file, bool dry_run, bool describe_json, bool print_pa
       if (done) break;
       t = t_got;
       out << "Converged in " << st.newton_iterations << " Newton iterations" << std::endl;
+      if (st.cond_number > 0) std::cout << "  (condition number estimate " << st.cond_number << ")" << std::endl;
       secs[1].calls += st.newton_iterations + 1;
       secs[1].time += st.time_residual_s;
       secs[2].calls += st.newton_iterations;

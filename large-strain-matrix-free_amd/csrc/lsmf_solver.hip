@@ -228,41 +228,6 @@ static void tridiag_extremes(const std::vector<double> &a, const std::vector<dou
   lmax = kth((int)n - 1);
 }
 
-// inverse of a symmetric positive definite matrix (row-major, in place) by
-// Cholesky factorisation; returns false if not positive definite
-static bool spd_inverse(std::vector<double> &A, int n) {
-  std::vector<double> Lm(A);
-  for (int j = 0; j < n; ++j) {
-    double d = Lm[(size_t)j * n + j];
-    for (int k = 0; k < j; ++k) d -= Lm[(size_t)j * n + k] * Lm[(size_t)j * n + k];
-    if (!(d > 0)) return false;
-    d = std::sqrt(d);
-    Lm[(size_t)j * n + j] = d;
-    for (int i = j + 1; i < n; ++i) {
-      double s = Lm[(size_t)i * n + j];
-      for (int k = 0; k < j; ++k) s -= Lm[(size_t)i * n + k] * Lm[(size_t)j * n + k];
-      Lm[(size_t)i * n + j] = s / d;
-    }
-  }
-  // inv(L) in place (lower), then A^-1 = inv(L)^T inv(L)
-  std::vector<double> Li((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) {
-    Li[(size_t)i * n + i] = 1.0 / Lm[(size_t)i * n + i];
-    for (int j = 0; j < i; ++j) {
-      double s = 0;
-      for (int k = j; k < i; ++k) s += Lm[(size_t)i * n + k] * Li[(size_t)k * n + j];
-      Li[(size_t)i * n + j] = -s / Lm[(size_t)i * n + i];
-    }
-  }
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j <= i; ++j) {
-      double s = 0;
-      for (int k = i; k < n; ++k) s += Li[(size_t)k * n + i] * Li[(size_t)k * n + j];
-      A[(size_t)i * n + j] = A[(size_t)j * n + i] = s;
-    }
-  return true;
-}
-
 struct GMGBase {
   virtual ~GMGBase() {}
   virtual void setup(const double *u_fine, double *tmp64a, double *tmp64b) = 0;
@@ -499,7 +464,8 @@ template <typename T> class GMG : public GMGBase {
     graph_r_ = nullptr;
   }
 
-  // FP64 coarse matrix by probing unit vectors, inverted on the host
+  // FP64 coarse matrix by probing unit vectors (column j = A e_j, stored as
+  // row j: A is symmetric), symmetrised and inverted in place on the device
   void assemble_coarse_inverse(const double *u0) {
     Level<T> &v = lv_[0];
     const int n = (int)v.g->n_dofs;
@@ -507,23 +473,27 @@ template <typename T> class GMG : public GMGBase {
       coarse64_->set_linearization(u0);
       coarse64_->cache();
     }
-    DeviceArray<double> e(n), col((size_t)n * n);
+    DeviceArray<double> e(n), piv(2 * (size_t)n);
+    DeviceArray<int> bad(1);
+    double *A = ainv_.p;
     for (int j = 0; j < n; ++j) {
       hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, s_, e.p, 0.0, (size_t)n);
       hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(1), 0, s_, e.p + j, 1.0, (size_t)1);
-      coarse64_->vmult(col.p + (size_t)j * n, e.p);
+      coarse64_->vmult(A + (size_t)j * n, e.p);
     }
-    std::vector<double> A((size_t)n * n);
-    hip_check(hipMemcpyAsync(A.data(), col.p, A.size() * sizeof(double), hipMemcpyDeviceToHost, s_), "coarse matrix");
-    hip_check(hipStreamSynchronize(s_), "coarse matrix");
-    // symmetrise (round-off) and invert
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < i; ++j) {
-        const double a = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
-        A[(size_t)i * n + j] = A[(size_t)j * n + i] = a;
-      }
-    if (!spd_inverse(A, n)) throw HipError(LSMF_ERR_NOT_CONVERGED, "coarse matrix is not positive definite");
-    ainv_.upload(A.data(), A.size());
+    const size_t nn = (size_t)n * n;
+    hipLaunchKernelGGL(k_sym_average, dim3(nblk(nn)), dim3(256), 0, s_, A, n);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(1), 0, s_, bad.p, 0, (size_t)1);
+    for (int k = 0; k < n; ++k) {
+      hipLaunchKernelGGL(k_gj_pivot, dim3(nblk(n)), dim3(256), 0, s_, (const double *)A, n, k, piv.p, piv.p + n,
+                         bad.p);
+      hipLaunchKernelGGL(k_gj_step, dim3(nblk(nn)), dim3(256), 0, s_, A, n, k, (const double *)piv.p,
+                         (const double *)(piv.p + n));
+    }
+    int bad_h = 0;
+    hip_check(hipMemcpyAsync(&bad_h, bad.p, sizeof(int), hipMemcpyDeviceToHost, s_), "coarse inverse");
+    hip_check(hipStreamSynchronize(s_), "coarse inverse");
+    if (bad_h) throw HipError(LSMF_ERR_NOT_CONVERGED, "coarse matrix is not positive definite");
     v.degree = 0;
     v.lmax_est = v.lmin_est = 0.0;
   }
@@ -813,6 +783,47 @@ void setup_preconditioner(lsmf_solver *S) {
     S->fine.compute_diagonal();
 }
 
+// SolverCG's condition-number estimate (include/mf_elasticity.h:3029-3046):
+// at most `its` unpreconditioned CG steps on A x = b from x = 0 (fewer when
+// the residual reaches tol), the Lanczos tridiagonal of the CG coefficients,
+// the ratio of its extreme eigenvalues
+double condition_number(lsmf_solver *S, const double *b, int its, double tol) {
+  const size_t n = S->n;
+  hipStream_t s = S->s;
+  Reducer &red = *S->red;
+  double *x = S->du.p, *r = S->r.p, *p = S->p.p, *Ap = S->Ap.p;
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, s, x, 0.0, n);
+  hip_check(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s), "cond copy");
+  hip_check(hipMemcpyAsync(p, b, n * sizeof(double), hipMemcpyDeviceToDevice, s), "cond copy");
+  double rr = red.dot(r, r, n);
+  const double r0 = std::sqrt(rr);
+  std::vector<double> alpha, beta;
+  for (int k = 0; k < its && r0 > tol; ++k) {
+    S->fine.vmult(Ap, p);
+    const double pAp = red.dot(p, Ap, n);
+    if (!(pAp > 0)) break;
+    const double a = rr / pAp;
+    alpha.push_back(a);
+    const double rr_new = red.cg_update(x, r, (const double *)p, (const double *)Ap, a, n);
+    if (std::sqrt(rr_new) <= tol) break;
+    const double bcoef = rr_new / rr;
+    beta.push_back(bcoef);
+    rr = rr_new;
+    hipLaunchKernelGGL(k_xpby<double>, dim3(nblk(n)), dim3(256), 0, s, p, (const double *)r, bcoef, n);
+  }
+  const size_t m = alpha.size();
+  if (m < 2) return m == 1 ? 1.0 : 0.0;
+  std::vector<double> diag(m), off(m - 1);
+  diag[0] = 1.0 / alpha[0];
+  for (size_t k = 1; k < m; ++k) {
+    diag[k] = 1.0 / alpha[k] + beta[k - 1] / alpha[k - 1];
+    off[k - 1] = std::sqrt(beta[k - 1]) / alpha[k - 1];
+  }
+  double lmin, lmax;
+  tridiag_extremes(diag, off, lmin, lmax);
+  return lmin > 0 ? lmax / lmin : 0.0;
+}
+
 // FP64 PCG (SolverCG::solve with the chosen preconditioner)
 int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, double *res_out) {
   const size_t n = S->n;
@@ -1094,6 +1105,7 @@ static void newton_step(lsmf_solver *S, double t, int step, lsmf_step_stats &st)
       if (res <= o.tol_f_abs || upd_prev <= o.tol_u_abs || (rn <= o.tol_f && un <= o.tol_u)) converged = true;
     }
     if (converged) break;
+    if (o.cond_number_cg_iterations > 0) st.cond_number = condition_number(S, S->rhs.p, o.cond_number_cg_iterations, o.tol_lin * res);
     t0 = Clock::now();
     setup_preconditioner(S);
     hip_check(hipStreamSynchronize(s), "setup sync");

@@ -197,6 +197,46 @@ __global__ void k_dense_matvec(const double *__restrict__ M, const T *__restrict
   if (lane == 0) x[row] = T(s);
 }
 
+// In-place inverse of a dense SPD matrix (row-major, n x n) by Gauss-Jordan
+// elimination without pivoting, one pivot per pair of launches: k_gj_pivot
+// saves row k and column k as they are before step k (and flags a pivot that
+// is not positive), k_gj_step applies step k to every entry. SPD keeps every
+// pivot positive, so no row exchanges are needed.
+__global__ void k_sym_average(double *__restrict__ A, int n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * n) return;
+  const int i = (int)(t / n), j = (int)(t % n);
+  if (i <= j) return;
+  const double a = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
+  A[(size_t)i * n + j] = a;
+  A[(size_t)j * n + i] = a;
+}
+
+__global__ void k_gj_pivot(const double *__restrict__ A, int n, int k, double *__restrict__ rowk,
+                           double *__restrict__ colk, int *__restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  rowk[i] = A[(size_t)k * n + i];
+  colk[i] = A[(size_t)i * n + k];
+  if (i == k && !(A[(size_t)k * n + k] > 0.0)) bad[0] = 1;
+}
+
+__global__ void k_gj_step(double *__restrict__ A, int n, int k, const double *__restrict__ rowk,
+                          const double *__restrict__ colk) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * n) return;
+  const int i = (int)(t / n), j = (int)(t % n);
+  const double pinv = 1.0 / rowk[k];
+  double v;
+  if (i == k)
+    v = j == k ? pinv : rowk[j] * pinv;
+  else if (j == k)
+    v = -colk[i] * pinv;
+  else
+    v = A[t] - colk[i] * rowk[j] * pinv;
+  A[t] = v;
+}
+
 // One banded 1D operator applied along direction `dir` of a node array of
 // shape (n2, n1, n0, ncomp) (x fastest); out has n_dir replaced by n_out:
 //   out[.., o, ..] = sum_k w[o * W + k] * in[.., start[o] + k, ..]

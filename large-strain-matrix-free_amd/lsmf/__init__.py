@@ -364,7 +364,8 @@ class SolverOptions(ctypes.Structure):
                 ("neumann_sides", ctypes.c_uint), ("neumann_traction", (ctypes.c_double * 3) * 6),
                 ("verbose", ctypes.c_int), ("coarse_solver", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("preconditioner", ctypes.c_int), ("preconditioner_relaxation", ctypes.c_double),
-                ("tol_f_abs", ctypes.c_double), ("tol_u_abs", ctypes.c_double), ("formulation", ctypes.c_int)]
+                ("tol_f_abs", ctypes.c_double), ("tol_u_abs", ctypes.c_double), ("formulation", ctypes.c_int),
+                ("cond_number_cg_iterations", ctypes.c_int)]
 
 
 COARSE = {"chebyshev": 0, "direct": 1}
@@ -376,7 +377,7 @@ class StepStats(ctypes.Structure):
                 ("max_linear_iterations", ctypes.c_int), ("residual_0", ctypes.c_double),
                 ("residual_final", ctypes.c_double), ("time_total_s", ctypes.c_double),
                 ("time_setup_s", ctypes.c_double), ("time_linear_s", ctypes.c_double),
-                ("time_residual_s", ctypes.c_double)]
+                ("time_residual_s", ctypes.c_double), ("cond_number", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -383,3 +383,42 @@ def test_cook_formulation0_newton_matches_blessed_output(name, pre, prec, cachin
         got = [float(f"{v:.6g}") for v in u[dim * node:dim * node + dim]]
         assert got == ref["displacement"], (k, got, ref["displacement"])
     assert S.step() is None
+
+
+def test_condition_number_estimate(device):
+    """the CG condition-number estimate run before each linear solve
+    (include/mf_elasticity.h:3029-3046): with enough iterations it equals
+    lambda_max / lambda_min of the tangent's free block (eigvalsh of the
+    oracle's tangent at the converged state); with 3 iterations the Ritz
+    values lie inside the spectrum, so the estimate is in [1, cond]; 0
+    iterations report 0"""
+    log("[cond] 6x6 Q1 unit square, x=0 clamped, u_x = 0.02 at x=1")
+    mesh = lsmf.Mesh(2, 1, [6, 6])
+    mu, lam = lsmf.lognormal_material(mesh.n_cells, 1.0, 0.3, 0.5, 7)
+    cons = mesh.constrained_mask({0: [0, 1], 1: [0]})
+
+    def solve(its):
+        o = lsmf.solver_options(dim=2, degree=1, n_q_1d=2, n_cells_coarse=[6, 6], n_levels=1, caching="tensor4",
+                                preconditioner="jacobi", tol_lin=1e-12, max_lin_iterations=1000, dt=1.0, t_end=1.0,
+                                cond_number_cg_iterations=its, dirichlet={0: ([0, 1], [0.0, 0.0]), 1: ([0], [0.02])})
+        S = lsmf.Solver(o, mu, lam)
+        stats = S.run()
+        assert len(stats) == 1
+        return stats[0]["cond_number"], S.solution()
+
+    est, u = solve(400)
+    ref = oracle.OracleOperator(2, 1, mesh.cell_nodes, mesh.coords, cons, caching="tensor4")
+    ref.set_material(mu, lam)
+    ref.set_linearization(u)
+    ref.cache()
+    K = np.stack([ref.vmult(e) for e in np.eye(mesh.n_dofs)], axis=1)
+    free = np.where(cons == 0)[0]
+    ev = np.linalg.eigvalsh(0.5 * (K + K.T)[np.ix_(free, free)])
+    exact = ev[-1] / ev[0]
+    log(f"  estimate {est:.10e} eigvalsh {exact:.10e}")
+    assert abs(est - exact) < 1e-6 * exact
+    est3, _ = solve(3)
+    log(f"  3 iterations: {est3:.6e}")
+    assert 1.0 <= est3 <= exact * (1 + 1e-10)
+    est0, _ = solve(0)
+    assert est0 == 0.0
