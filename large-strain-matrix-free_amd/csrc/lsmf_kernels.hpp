@@ -336,6 +336,191 @@ template <int DIM, int N, typename T> struct SF {
 };
 
 // ------------------------------------------------------------------------
+// 3D sum factorization with one lane per 1D line.  In each of the three
+// passes a lane takes one line (slot, component, line of N points along the
+// pass direction), reads its values from LDS into registers, applies the 1D
+// matrices and writes the results back over the same line.  The matrices
+// are the same for every lane, so they come from the kernel arguments
+// (scalar registers), not from LDS.  Per point and pass that is one LDS read
+// and one write per array, where one lane per point reads the N line values
+// and N matrix entries for each value it writes.
+// Every pass reads and writes only the positions of the lines it owns, so
+// the passes work in place: slot s owns one buffer lds + s * SLOT of NB = 9
+// arrays of NQP values, array e * 3 + c holding stage value e of component
+// c.  sf holds S[q][k] then D[q][k] (N x N each).
+// ------------------------------------------------------------------------
+template <int N, typename T> struct SFLine {
+  static constexpr int NQP = N * N * N, NL = N * N, NB = 9, SLOT = NB * NQP;
+
+  // y[q] = sum_k M[q][k] x[k] (MT: M[k][q])
+  template <bool MT>
+  static __device__ __forceinline__ T row(const T *M, int q, const T (&x)[N]) {
+    T s = M[MT ? q : q * N] * x[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) s = fma(M[MT ? k * N + q : q * N + k], x[k], s);
+    return s;
+  }
+
+  // task -> slot s, component c, line l; p = first array of (s, c)
+  static __device__ __forceinline__ T *task_base(T *lds, int task, int &l) {
+    const int s = task / (3 * NL), rem = task - s * 3 * NL, c = rem / NL;
+    l = rem - c * NL;
+    return lds + s * SLOT + c * NQP;
+  }
+
+  // nodal values in array c (c < 3) of slots s < CPW -> gradients G[c][d]
+  // at quadrature point r of this lane's slot (lanes with on), left in
+  // array d * 3 + c.  A lane reads only its own point r at the end, so the
+  // caller may overwrite point r of any array (integrate) without a barrier.
+  template <int CPW>
+  static __device__ __forceinline__ void evaluate(const T *sf, T *lds, int t, bool on, int slot, int r,
+                                                  T (&G)[3][3]) {
+    const T *S = sf, *D = sf + N * N;
+    constexpr int TASKS = CPW * 3 * NL;
+    constexpr int E1 = 3 * NQP, E2 = 6 * NQP;  // offsets of stage arrays 1, 2
+    // x: e0 = S u, e1 = D u along x; line l = (i1, i2)
+#pragma unroll 1
+    for (int task = t; task < TASKS; task += kWG) {
+      int l;
+      T *p = task_base(lds, task, l) + l * N;
+      T x[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) x[k] = p[k];
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        p[q] = row<false>(S, q, x);
+        p[E1 + q] = row<false>(D, q, x);
+      }
+    }
+    lds_barrier();
+    // y: e0 <- S e0 (value), e1 <- S e1 (d/dx), e2 = D e0 (d/dy); line (i0, i2)
+#pragma unroll 1
+    for (int task = t; task < TASKS; task += kWG) {
+      int l;
+      T *p = task_base(lds, task, l) + (l % N) + NL * (l / N);
+      T x0[N], x1[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        x0[k] = p[k * N];
+        x1[k] = p[E1 + k * N];
+      }
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        p[q * N] = row<false>(S, q, x0);
+        p[E1 + q * N] = row<false>(S, q, x1);
+        p[E2 + q * N] = row<false>(D, q, x0);
+      }
+    }
+    lds_barrier();
+    // z: G_c0 = S e1, G_c1 = S e2, G_c2 = D e0 into arrays e0, e1, e2; line (i0, i1)
+#pragma unroll 1
+    for (int task = t; task < TASKS; task += kWG) {
+      int l;
+      T *p = task_base(lds, task, l) + l;
+      T y0[N], y1[N], y2[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        y0[k] = p[k * NL];
+        y1[k] = p[E1 + k * NL];
+        y2[k] = p[E2 + k * NL];
+      }
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        p[q * NL] = row<false>(S, q, y1);
+        p[E1 + q * NL] = row<false>(S, q, y2);
+        p[E2 + q * NL] = row<false>(D, q, y0);
+      }
+    }
+    lds_barrier();
+    if (on) {
+      const T *g = lds + slot * SLOT + r;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) G[c][d] = g[(3 * d + c) * NQP];
+    }
+  }
+
+  // fluxes Phi[c][d] at quadrature point r of this lane's slot -> out[c] =
+  // sum_q sum_d dphi_r/dxi_d Phi_cd at support point r (read back from
+  // array c after the last barrier).
+  template <int CPW>
+  static __device__ __forceinline__ void integrate(const T *sf, T *lds, int t, bool on, int slot, int r,
+                                                   const T (&Phi)[3][3], T (&out)[3]) {
+    const T *S = sf, *D = sf + N * N;
+    constexpr int TASKS = CPW * 3 * NL;
+    constexpr int E1 = 3 * NQP, E2 = 6 * NQP;
+    if (on) {
+      T *w = lds + slot * SLOT + r;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) w[(3 * d + c) * NQP] = Phi[c][d];
+    }
+    lds_barrier();
+    // z^T: e0 <- S^T e0, e1 <- S^T e1, e2 <- D^T e2; line (i0, i1)
+#pragma unroll 1
+    for (int task = t; task < TASKS; task += kWG) {
+      int l;
+      T *p = task_base(lds, task, l) + l;
+      T f0[N], f1[N], f2[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        f0[k] = p[k * NL];
+        f1[k] = p[E1 + k * NL];
+        f2[k] = p[E2 + k * NL];
+      }
+#pragma unroll
+      for (int a = 0; a < N; ++a) {
+        p[a * NL] = row<true>(S, a, f0);
+        p[E1 + a * NL] = row<true>(S, a, f1);
+        p[E2 + a * NL] = row<true>(D, a, f2);
+      }
+    }
+    lds_barrier();
+    // y^T: e0 <- S^T e2 + D^T e1, e1 <- S^T e0; line (i0, i2)
+#pragma unroll 1
+    for (int task = t; task < TASKS; task += kWG) {
+      int l;
+      T *p = task_base(lds, task, l) + (l % N) + NL * (l / N);
+      T r10[N], r01[N], r00[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        r10[k] = p[k * N];
+        r01[k] = p[E1 + k * N];
+        r00[k] = p[E2 + k * N];
+      }
+#pragma unroll
+      for (int a = 0; a < N; ++a) {
+        p[a * N] = row<true>(S, a, r00) + row<true>(D, a, r01);
+        p[E1 + a * N] = row<true>(S, a, r10);
+      }
+    }
+    lds_barrier();
+    // x^T: e0 <- S^T e0 + D^T e1; line (i1, i2)
+#pragma unroll 1
+    for (int task = t; task < TASKS; task += kWG) {
+      int l;
+      T *p = task_base(lds, task, l) + l * N;
+      T p0[N], p1[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        p0[k] = p[k];
+        p1[k] = p[E1 + k];
+      }
+#pragma unroll
+      for (int a = 0; a < N; ++a) p[a] = row<true>(S, a, p0) + row<true>(D, a, p1);
+    }
+    lds_barrier();
+    if (on) {
+      const T *v = lds + slot * SLOT + r;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) out[c] = v[c * NQP];
+    }
+  }
+};
+
+// ------------------------------------------------------------------------
 // pointwise tangent actions (reference: include/mf_nh_operator.h:1182-1440)
 // ------------------------------------------------------------------------
 // g = G K
@@ -627,15 +812,21 @@ template <typename Number> struct VmultParams {
   double *__restrict__ dmu;
   double *__restrict__ dlambda;
   const int32_t *__restrict__ perm;
+  // 1D matrices S[q][k], D[q][k] (N <= 5) for the line-per-lane sum
+  // factorization: uniform across lanes, read from the kernel arguments
+  Number sf[2 * 25];
 };
 
 // Kinematic prologue shared by the scalar vmult and the residual: gathers u
 // and the corner coordinates, returns F = I + Grad_xi u J_ref^-1 and
 // J_ref^-1, det(J_ref) at this lane's quadrature point.  Leaves A,B free.
-template <int DIM, int N, typename T>
+// LINE: 3D gradient by the line-per-lane sum factorization (P.sf set,
+// lds = the workgroup's slot buffers, t = thread, slot = this lane's cell).
+template <int DIM, int N, typename T, bool LINE = false>
 __device__ __forceinline__ void kinematics_prologue(const VmultParams<T> &P, const T *tab, T *A, T *B, T *Xc,
                                                     bool on, bool active, uint32_t code, int r, int i0, int i1, int i2,
-                                                    T (&F)[DIM][DIM], T (&Jri)[DIM][DIM], T &detJr) {
+                                                    T (&F)[DIM][DIM], T (&Jri)[DIM][DIM], T &detJr,
+                                                    T *lds = nullptr, int t = 0, int slot = 0) {
   constexpr int NQP = ipow(N, DIM);
   const int node = code & kNodeMask;
   if (active) {
@@ -648,7 +839,10 @@ __device__ __forceinline__ void kinematics_prologue(const VmultParams<T> &P, con
   }
   lds_barrier();
   T Gu[DIM][DIM];
-  SF<DIM, N, T>::template evaluate<DIM>(tab, A, B, on, r, i0, i1, i2, Gu);
+  if constexpr (LINE && DIM == 3)
+    SFLine<N, T>::template evaluate<CellShape<DIM, N>::CPW>(P.sf, lds, t, on, slot, r, Gu);
+  else
+    SF<DIM, N, T>::template evaluate<DIM>(tab, A, B, on, r, i0, i1, i2, Gu);
   T Jr[DIM][DIM];
   q1_jacobian<DIM, T, T>(Xc, tab + 4 * N * N, i0, i1, i2, Jr);
   detJr = inverse<DIM>(Jr, Jri);
@@ -726,10 +920,12 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   constexpr bool KIN = (VAR == V_SCALAR || VAR == V_SCALAR_REF || RECOMP);
   constexpr int TABN = tab_size<N>(VAR);
   constexpr int RED = (VAR == V_ADJOINT) ? CPW * 2 * NQP : 0;  // per-cell reductions
-  __shared__ T lds[CPW * 2 * NB * NQP + TABN + (KIN ? CPW * NC * DIM : 0) + RED];
+  // per-cell LDS buffers: 3D sums in place (SFLine, NB arrays), 2D ping-pongs
+  constexpr int SLOTW = (DIM == 3 ? 1 : 2) * NB * NQP;
+  __shared__ T lds[CPW * SLOTW + TABN + (KIN ? CPW * NC * DIM : 0) + RED];
   __shared__ T tile_v[TILE ? TN * DIM : 1];         // tile nodes' source values
   __shared__ uint32_t tile_c[TILE ? TN : 1];        // tile nodes' codes
-  T *tab = lds + CPW * 2 * NB * NQP;
+  T *tab = lds + CPW * SLOTW;
   T *Xall = tab + TABN;
   T *Rall = Xall + (KIN ? CPW * NC * DIM : 0);
 
@@ -740,8 +936,8 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   const int i0 = r % N, i1 = (r / N) % N, i2 = (DIM == 3) ? r / (N * N) : 0;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const bool on = slot < CPW;
-  T *A = lds + (slot < CPW ? slot : 0) * 2 * NB * NQP;
-  T *B = A + NB * NQP;
+  T *A = lds + (slot < CPW ? slot : 0) * SLOTW;
+  T *B = DIM == 3 ? A : A + NB * NQP;  // 2D sum factorization only
   T *Xc = Xall + (slot < CPW ? slot : 0) * NC * DIM;
 
   // Load order matters: vmcnt is an in-order counter, so consuming a load
@@ -821,7 +1017,8 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
     for (int c = 0; c < DIM; ++c) uq[c] = cch[1 + c];
     kinematics_collocation<DIM, N, T>(P, tab, A, Xc, on, active, code, r, i0, i1, i2, uq, F, Jri, detJr);
   } else if constexpr (KIN && (PH & PH_QD)) {
-    kinematics_prologue<DIM, N, T>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr);
+    kinematics_prologue<DIM, N, T, true>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr, lds, t,
+                                         slot);
   }
 
   T G[DIM][DIM];
@@ -858,7 +1055,10 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) out[c] = A[c * NQP + r];
     } else {
-      SF<DIM, N, T>::template evaluate<DIM>(tab, A, B, on, r, i0, i1, i2, G);
+      if constexpr (DIM == 3)
+        SFLine<N, T>::template evaluate<CPW>(P.sf, lds, t, on, slot, r, G);
+      else
+        SF<DIM, N, T>::template evaluate<DIM>(tab, A, B, on, r, i0, i1, i2, G);
     }
   }
 
@@ -987,7 +1187,7 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
       }  // formulation 1
     }
     if constexpr (DIM == 3)
-      SF<DIM, N, T>::integrate(tab, B, A, on, r, i0, i1, i2, Phi, out);
+      SFLine<N, T>::template integrate<CPW>(P.sf, lds, t, on, slot, r, Phi, out);
     else
       SF<DIM, N, T>::integrate(tab, A, B, on, r, i0, i1, i2, Phi, out);
     if constexpr (VAR == V_ADJOINT) {

@@ -34,6 +34,8 @@ template <int DIM, int N> struct Launch {
     P.dmu = op->adj_dmu;
     P.dlambda = op->adj_dlambda;
     P.perm = op->d_perm;
+    static_assert(sizeof(P.sf) / sizeof(P.sf[0]) >= 2 * N * N, "1D matrices do not fit the kernel arguments");
+    for (int i = 0; i < 2 * N * N; ++i) P.sf[i] = T(op->h_sf[i]);
     return P;
   }
 

@@ -481,6 +481,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
     tab[4 * N * N + q] = t1.qpts[q];
     tab[4 * N * N + N + q] = t1.qw[q];
   }
+  for (int i = 0; i < 2 * N * N; ++i) op->h_sf[i] = tab[i];
 
   const size_t es = elem_size(op);
   int rc = 0;

@@ -62,6 +62,7 @@ struct lsmf_op {
   double *d_coords = nullptr;       // [n_nodes][dim]
   void *d_tab = nullptr;            // S, D, St, Dt, qp, qw in operator precision
   double *d_tab64 = nullptr;        // same tables in FP64 (cache kernel)
+  double h_sf[2 * 25] = {};         // S[q][k] then D[q][k] (N x N each), kernel-argument copy
   double *d_mu64 = nullptr, *d_lambda64 = nullptr;  // per sorted cell
   void *d_mu = nullptr, *d_lambda = nullptr;        // operator precision
   double *d_u64 = nullptr;          // linearisation state copy
