@@ -336,36 +336,120 @@ template <int DIM, int N, typename T> struct SF {
 };
 
 // ------------------------------------------------------------------------
+// Even-odd form of a 1D matrix M (N x N) that is centrosymmetric (SIG = +1:
+// M[N-1-q][N-1-k] = M[q][k], the interpolation S between symmetric point
+// sets) or centro-antisymmetric (SIG = -1: the derivative D).  With
+// x_e[k] = x[k] + x[N-1-k], x_o[k] = x[k] - x[N-1-k] (k < H = N / 2) and the
+// middle value x_m (odd N), M x and M^T x need only the H-column halves
+//   E[r][k] = (M[r][k] + M[r][N-1-k]) / 2,  O[r][k] = (M[r][k] - M[r][N-1-k]) / 2
+// and the middle column C[r] = M[r][H], rows r < R = H + N % 2: about half
+// the multiply-adds and half the constants of the plain products.
+// Packed per matrix: E[R][H], O[R][H], C[R] (PK values).
+// ------------------------------------------------------------------------
+template <int N, typename T> struct EvenOdd {
+  static constexpr int H = N / 2, ODD = N % 2, R = H + ODD, PK = R * (2 * H + 1);
+  struct Split {
+    T e[H], o[H], m;
+  };
+  static __device__ __forceinline__ Split split(const T (&x)[N]) {
+    Split s;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      s.e[k] = x[k] + x[N - 1 - k];
+      s.o[k] = x[k] - x[N - 1 - k];
+    }
+    s.m = ODD ? x[H] : T(0);
+    return s;
+  }
+  static __device__ __forceinline__ T E(const T *pk, int r, int k) { return pk[r * H + k]; }
+  static __device__ __forceinline__ T O(const T *pk, int r, int k) { return pk[R * H + r * H + k]; }
+  static __device__ __forceinline__ T C(const T *pk, int r) { return pk[2 * R * H + r]; }
+
+  // y = M x
+  template <int SIG> static __device__ __forceinline__ void fwd(const T *pk, const Split &s, T (&y)[N]) {
+#pragma unroll
+    for (int q = 0; q < H; ++q) {
+      T ev = ODD ? C(pk, q) * s.m : T(0), od = T(0);
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        ev = fma(E(pk, q, k), s.e[k], ev);
+        od = fma(O(pk, q, k), s.o[k], od);
+      }
+      y[q] = ev + od;
+      y[N - 1 - q] = SIG > 0 ? ev - od : od - ev;
+    }
+    if constexpr (ODD) {
+      T v = SIG > 0 ? C(pk, H) * s.m : T(0);
+#pragma unroll
+      for (int k = 0; k < H; ++k) v = SIG > 0 ? fma(E(pk, H, k), s.e[k], v) : fma(O(pk, H, k), s.o[k], v);
+      y[H] = v;
+    }
+  }
+  // y = M^T x
+  template <int SIG> static __device__ __forceinline__ void tr(const T *pk, const Split &s, T (&y)[N]) {
+#pragma unroll
+    for (int a = 0; a < H; ++a) {
+      T u, w;
+      if constexpr (SIG > 0) {
+        u = ODD ? E(pk, H, a) * s.m : T(0);  // even part
+        w = T(0);                            // odd part
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+          u = fma(E(pk, q, a), s.e[q], u);
+          w = fma(O(pk, q, a), s.o[q], w);
+        }
+      } else {
+        u = T(0);
+        w = ODD ? O(pk, H, a) * s.m : T(0);
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+          u = fma(E(pk, q, a), s.o[q], u);
+          w = fma(O(pk, q, a), s.e[q], w);
+        }
+      }
+      y[a] = u + w;
+      y[N - 1 - a] = u - w;
+    }
+    if constexpr (ODD) {
+      T v = SIG > 0 ? C(pk, H) * s.m : T(0);
+#pragma unroll
+      for (int q = 0; q < H; ++q) v = fma(C(pk, q), SIG > 0 ? s.e[q] : s.o[q], v);
+      y[H] = v;
+    }
+  }
+};
+
+// ------------------------------------------------------------------------
 // 3D sum factorization with one lane per 1D line.  In each of the three
 // passes a lane takes one line (slot, component, line of N points along the
 // pass direction), reads its values from LDS into registers, applies the 1D
-// matrices and writes the results back over the same line.  The matrices
-// are the same for every lane, so they come from the kernel arguments
-// (scalar registers), not from LDS.  Per point and pass that is one LDS read
-// and one write per array, where one lane per point reads the N line values
-// and N matrix entries for each value it writes.
+// matrices in even-odd form and writes the results back over the same line.
+// The matrices are the same for every lane, so they come from the kernel
+// arguments (scalar registers), not from LDS.  Per point and pass that is
+// one LDS read and one write per array, where one lane per point reads the
+// N line values and N matrix entries for each value it writes.
 // Every pass reads and writes only the positions of the lines it owns, so
 // the passes work in place: slot s owns one buffer lds + s * SLOT of NB = 9
 // arrays of NQP values, array e * 3 + c holding stage value e of component
-// c.  sf holds S[q][k] then D[q][k] (N x N each).
+// c.  sf holds the packed even-odd forms of S, then of D (EvenOdd::PK each).
 // ------------------------------------------------------------------------
 template <int N, typename T> struct SFLine {
   static constexpr int NQP = N * N * N, NL = N * N, NB = 9, SLOT = NB * NQP;
+  using EO = EvenOdd<N, T>;
 
-  // y[q] = sum_k M[q][k] x[k] (MT: M[k][q])
-  template <bool MT>
-  static __device__ __forceinline__ T row(const T *M, int q, const T (&x)[N]) {
-    T s = M[MT ? q : q * N] * x[0];
-#pragma unroll
-    for (int k = 1; k < N; ++k) s = fma(M[MT ? k * N + q : q * N + k], x[k], s);
-    return s;
-  }
-
-  // task -> slot s, component c, line l; p = first array of (s, c)
+  // task -> slot s, component c, line l; returns the first array of (s, c)
   static __device__ __forceinline__ T *task_base(T *lds, int task, int &l) {
     const int s = task / (3 * NL), rem = task - s * 3 * NL, c = rem / NL;
     l = rem - c * NL;
     return lds + s * SLOT + c * NQP;
+  }
+  template <int STRIDE> static __device__ __forceinline__ void load(const T *p, T (&x)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[k] = p[k * STRIDE];
+  }
+  template <int STRIDE> static __device__ __forceinline__ void store(T *p, const T (&x)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) p[k * STRIDE] = x[k];
   }
 
   // nodal values in array c (c < 3) of slots s < CPW -> gradients G[c][d]
@@ -375,7 +459,7 @@ template <int N, typename T> struct SFLine {
   template <int CPW>
   static __device__ __forceinline__ void evaluate(const T *sf, T *lds, int t, bool on, int slot, int r,
                                                   T (&G)[3][3]) {
-    const T *S = sf, *D = sf + N * N;
+    const T *S = sf, *D = sf + EO::PK;
     constexpr int TASKS = CPW * 3 * NL;
     constexpr int E1 = 3 * NQP, E2 = 6 * NQP;  // offsets of stage arrays 1, 2
     // x: e0 = S u, e1 = D u along x; line l = (i1, i2)
@@ -383,14 +467,13 @@ template <int N, typename T> struct SFLine {
     for (int task = t; task < TASKS; task += kWG) {
       int l;
       T *p = task_base(lds, task, l) + l * N;
-      T x[N];
-#pragma unroll
-      for (int k = 0; k < N; ++k) x[k] = p[k];
-#pragma unroll
-      for (int q = 0; q < N; ++q) {
-        p[q] = row<false>(S, q, x);
-        p[E1 + q] = row<false>(D, q, x);
-      }
+      T x[N], y[N];
+      load<1>(p, x);
+      const typename EO::Split sx = EO::split(x);
+      EO::template fwd<1>(S, sx, y);
+      store<1>(p, y);
+      EO::template fwd<-1>(D, sx, y);
+      store<1>(p + E1, y);
     }
     lds_barrier();
     // y: e0 <- S e0 (value), e1 <- S e1 (d/dx), e2 = D e0 (d/dy); line (i0, i2)
@@ -398,18 +481,16 @@ template <int N, typename T> struct SFLine {
     for (int task = t; task < TASKS; task += kWG) {
       int l;
       T *p = task_base(lds, task, l) + (l % N) + NL * (l / N);
-      T x0[N], x1[N];
-#pragma unroll
-      for (int k = 0; k < N; ++k) {
-        x0[k] = p[k * N];
-        x1[k] = p[E1 + k * N];
-      }
-#pragma unroll
-      for (int q = 0; q < N; ++q) {
-        p[q * N] = row<false>(S, q, x0);
-        p[E1 + q * N] = row<false>(S, q, x1);
-        p[E2 + q * N] = row<false>(D, q, x0);
-      }
+      T x0[N], x1[N], y[N];
+      load<N>(p, x0);
+      load<N>(p + E1, x1);
+      const typename EO::Split s0 = EO::split(x0), s1 = EO::split(x1);
+      EO::template fwd<1>(S, s0, y);
+      store<N>(p, y);
+      EO::template fwd<-1>(D, s0, y);
+      store<N>(p + E2, y);
+      EO::template fwd<1>(S, s1, y);
+      store<N>(p + E1, y);
     }
     lds_barrier();
     // z: G_c0 = S e1, G_c1 = S e2, G_c2 = D e0 into arrays e0, e1, e2; line (i0, i1)
@@ -417,19 +498,16 @@ template <int N, typename T> struct SFLine {
     for (int task = t; task < TASKS; task += kWG) {
       int l;
       T *p = task_base(lds, task, l) + l;
-      T y0[N], y1[N], y2[N];
-#pragma unroll
-      for (int k = 0; k < N; ++k) {
-        y0[k] = p[k * NL];
-        y1[k] = p[E1 + k * NL];
-        y2[k] = p[E2 + k * NL];
-      }
-#pragma unroll
-      for (int q = 0; q < N; ++q) {
-        p[q * NL] = row<false>(S, q, y1);
-        p[E1 + q * NL] = row<false>(S, q, y2);
-        p[E2 + q * NL] = row<false>(D, q, y0);
-      }
+      T y0[N], y1[N], y2[N], y[N];
+      load<NL>(p, y0);
+      load<NL>(p + E1, y1);
+      load<NL>(p + E2, y2);
+      EO::template fwd<1>(S, EO::split(y1), y);
+      store<NL>(p, y);
+      EO::template fwd<1>(S, EO::split(y2), y);
+      store<NL>(p + E1, y);
+      EO::template fwd<-1>(D, EO::split(y0), y);
+      store<NL>(p + E2, y);
     }
     lds_barrier();
     if (on) {
@@ -447,7 +525,7 @@ template <int N, typename T> struct SFLine {
   template <int CPW>
   static __device__ __forceinline__ void integrate(const T *sf, T *lds, int t, bool on, int slot, int r,
                                                    const T (&Phi)[3][3], T (&out)[3]) {
-    const T *S = sf, *D = sf + N * N;
+    const T *S = sf, *D = sf + EO::PK;
     constexpr int TASKS = CPW * 3 * NL;
     constexpr int E1 = 3 * NQP, E2 = 6 * NQP;
     if (on) {
@@ -463,19 +541,16 @@ template <int N, typename T> struct SFLine {
     for (int task = t; task < TASKS; task += kWG) {
       int l;
       T *p = task_base(lds, task, l) + l;
-      T f0[N], f1[N], f2[N];
-#pragma unroll
-      for (int k = 0; k < N; ++k) {
-        f0[k] = p[k * NL];
-        f1[k] = p[E1 + k * NL];
-        f2[k] = p[E2 + k * NL];
-      }
-#pragma unroll
-      for (int a = 0; a < N; ++a) {
-        p[a * NL] = row<true>(S, a, f0);
-        p[E1 + a * NL] = row<true>(S, a, f1);
-        p[E2 + a * NL] = row<true>(D, a, f2);
-      }
+      T f0[N], f1[N], f2[N], y[N];
+      load<NL>(p, f0);
+      load<NL>(p + E1, f1);
+      load<NL>(p + E2, f2);
+      EO::template tr<1>(S, EO::split(f0), y);
+      store<NL>(p, y);
+      EO::template tr<1>(S, EO::split(f1), y);
+      store<NL>(p + E1, y);
+      EO::template tr<-1>(D, EO::split(f2), y);
+      store<NL>(p + E2, y);
     }
     lds_barrier();
     // y^T: e0 <- S^T e2 + D^T e1, e1 <- S^T e0; line (i0, i2)
@@ -483,18 +558,17 @@ template <int N, typename T> struct SFLine {
     for (int task = t; task < TASKS; task += kWG) {
       int l;
       T *p = task_base(lds, task, l) + (l % N) + NL * (l / N);
-      T r10[N], r01[N], r00[N];
+      T r10[N], r01[N], r00[N], y[N], z[N];
+      load<N>(p, r10);
+      load<N>(p + E1, r01);
+      load<N>(p + E2, r00);
+      EO::template tr<1>(S, EO::split(r00), y);
+      EO::template tr<-1>(D, EO::split(r01), z);
 #pragma unroll
-      for (int k = 0; k < N; ++k) {
-        r10[k] = p[k * N];
-        r01[k] = p[E1 + k * N];
-        r00[k] = p[E2 + k * N];
-      }
-#pragma unroll
-      for (int a = 0; a < N; ++a) {
-        p[a * N] = row<true>(S, a, r00) + row<true>(D, a, r01);
-        p[E1 + a * N] = row<true>(S, a, r10);
-      }
+      for (int a = 0; a < N; ++a) y[a] += z[a];
+      store<N>(p, y);
+      EO::template tr<1>(S, EO::split(r10), y);
+      store<N>(p + E1, y);
     }
     lds_barrier();
     // x^T: e0 <- S^T e0 + D^T e1; line (i1, i2)
@@ -502,14 +576,14 @@ template <int N, typename T> struct SFLine {
     for (int task = t; task < TASKS; task += kWG) {
       int l;
       T *p = task_base(lds, task, l) + l * N;
-      T p0[N], p1[N];
+      T p0[N], p1[N], y[N], z[N];
+      load<1>(p, p0);
+      load<1>(p + E1, p1);
+      EO::template tr<1>(S, EO::split(p0), y);
+      EO::template tr<-1>(D, EO::split(p1), z);
 #pragma unroll
-      for (int k = 0; k < N; ++k) {
-        p0[k] = p[k];
-        p1[k] = p[E1 + k];
-      }
-#pragma unroll
-      for (int a = 0; a < N; ++a) p[a] = row<true>(S, a, p0) + row<true>(D, a, p1);
+      for (int a = 0; a < N; ++a) y[a] += z[a];
+      store<1>(p, y);
     }
     lds_barrier();
     if (on) {
@@ -812,9 +886,10 @@ template <typename Number> struct VmultParams {
   double *__restrict__ dmu;
   double *__restrict__ dlambda;
   const int32_t *__restrict__ perm;
-  // 1D matrices S[q][k], D[q][k] (N <= 5) for the line-per-lane sum
-  // factorization: uniform across lanes, read from the kernel arguments
-  Number sf[2 * 25];
+  // even-odd forms of the 1D matrices S and D (EvenOdd<N>::PK values each,
+  // N <= 5) for the line-per-lane sum factorization: uniform across lanes,
+  // read from the kernel arguments
+  Number sf[2 * 15];
 };
 
 // Kinematic prologue shared by the scalar vmult and the residual: gathers u
@@ -958,6 +1033,9 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   bool tvalid = false;
   size_t tnode = 0;
   T sv[DIM];
+  T dv[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) dv[c] = T(0);
   if constexpr (TILE) {
     const int first = P.tinfo[2 * (size_t)tile], ext = P.tinfo[2 * (size_t)tile + 1];
     const int ex = ext & 0xff, ey = (ext >> 8) & 0xff, ez = ext >> 16;
@@ -971,6 +1049,12 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) sv[c] = P.src[tnode * DIM + c];
     }
+    // the node's destination values, read now rather than at the final
+    // read-modify-write: no other tile of this colour touches the node, so
+    // they do not change before this tile writes them
+    if (tvalid && (ADD || !(tcode & kFirstBit)))
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) dv[c] = P.dst[tnode * DIM + c];
   } else {
     cell = P.cell_begin + L * CPW + slot;
     active = on && (cell < P.cell_end);
@@ -1235,17 +1319,10 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
 #pragma unroll
             for (int c = 0; c < DIM; ++c) acc[c] += res[lane * DIM + c];
           }
-      const bool first_touch = !ADD && (tcode & kFirstBit);
-      T old[DIM];
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        const bool skip = CONSTR && ((tcode >> (kConsShift + c)) & 1u);
-        old[c] = (first_touch || skip) ? T(0) : P.dst[tnode * DIM + c];
-      }
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
         if (CONSTR && ((tcode >> (kConsShift + c)) & 1u)) continue;
-        P.dst[tnode * DIM + c] = old[c] + acc[c];
+        P.dst[tnode * DIM + c] = dv[c] + acc[c];  // dv = 0 on first touch
       }
     }
     return;

@@ -34,8 +34,20 @@ template <int DIM, int N> struct Launch {
     P.dmu = op->adj_dmu;
     P.dlambda = op->adj_dlambda;
     P.perm = op->d_perm;
-    static_assert(sizeof(P.sf) / sizeof(P.sf[0]) >= 2 * N * N, "1D matrices do not fit the kernel arguments");
-    for (int i = 0; i < 2 * N * N; ++i) P.sf[i] = T(op->h_sf[i]);
+    using EO = EvenOdd<N, T>;
+    static_assert(sizeof(P.sf) / sizeof(P.sf[0]) >= 2 * EO::PK, "1D matrices do not fit the kernel arguments");
+    // even-odd forms of S (rows r < R) and D, from the plain tables
+    for (int m = 0; m < 2; ++m) {
+      const double *M = op->h_sf + m * N * N;
+      T *pk = P.sf + m * EO::PK;
+      for (int r = 0; r < EO::R; ++r) {
+        for (int k = 0; k < EO::H; ++k) {
+          pk[r * EO::H + k] = T(0.5 * (M[r * N + k] + M[r * N + N - 1 - k]));
+          pk[EO::R * EO::H + r * EO::H + k] = T(0.5 * (M[r * N + k] - M[r * N + N - 1 - k]));
+        }
+        pk[2 * EO::R * EO::H + r] = EO::ODD ? T(M[r * N + EO::H]) : T(0);
+      }
+    }
     return P;
   }
 
