@@ -875,6 +875,7 @@ template <typename Number> struct VmultParams {
   const Number *__restrict__ lambda;
   int cell_begin, cell_end;  // sorted cells of one colour, or its tiles (tiled layout)
   int formulation;  // 1, or 0: the volumetric/isochoric split (scalar, tensor4, residual)
+  int curved;       // support points off the Q1 maps: Eulerian Q_p metric (scalar variants)
   const uint32_t *__restrict__ tnodes;  // tiled layout: [tile][TN] node codes
   const int32_t *__restrict__ tinfo;    // tiled layout: [tile][2] first cell, extents
   // node (hanging) constraints, cells layout only; hang == nullptr: none
@@ -895,12 +896,51 @@ template <typename Number> struct VmultParams {
 // Kinematic prologue shared by the scalar vmult and the residual: gathers u
 // and the corner coordinates, returns F = I + Grad_xi u J_ref^-1 and
 // J_ref^-1, det(J_ref) at this lane's quadrature point.  Leaves A,B free.
+// Meshes whose support points do not all lie on their cells' Q1 maps
+// ("curved"): the current configuration is the Eulerian Q_p mapping
+// x(xi) = sum_a (X_a + u_a) phi_a(xi) (MappingQEulerian of degree p,
+// include/mf_elasticity.h:1821-1831), so its Jacobian is Grad_xi X + Grad_xi u,
+// while the reference quantities (F, the residual) keep the Q1 vertex map
+// (StaticMappingQ1, :1828-1829).  Returns GX = Grad_xi X at this lane's
+// quadrature point from the support-point coordinates; A (and B for the
+// ping-pong sum factorization) are free again on return.
+template <int DIM, int N, typename T, bool LINE>
+__device__ __forceinline__ void geometry_gradient(const double *__restrict__ coords, const T *sf, const T *tab,
+                                                  T *lds, T *A, T *B, bool on, bool active, uint32_t code, int t,
+                                                  int slot, int r, int i0, int i1, int i2, T (&GX)[DIM][DIM]) {
+  constexpr int NQP = ipow(N, DIM);
+  const size_t node = code & kNodeMask;
+  if (active)
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) A[c * NQP + r] = T(coords[node * DIM + c]);
+  lds_barrier();
+  if constexpr (LINE && DIM == 3)
+    SFLine<N, T>::template evaluate<CellShape<DIM, N>::CPW>(sf, lds, t, on, slot, r, GX);
+  else
+    SF<DIM, N, T>::template evaluate<DIM>(tab, A, B, on, r, i0, i1, i2, GX);
+  lds_barrier();
+}
+
+// current metric of a curved mesh: K = (GX + Gu)^-1 and the integration
+// factor JxW_cur / J = det(GX + Gu) w / J
+template <int DIM, typename T>
+__device__ __forceinline__ void eulerian_metric(const T (&GX)[DIM][DIM], const T (&Gu)[DIM][DIM], T J, T w,
+                                                T (&K)[DIM][DIM], T &dw) {
+  T Jc[DIM][DIM];
+#pragma unroll
+  for (int i = 0; i < DIM; ++i)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) Jc[i][d] = GX[i][d] + Gu[i][d];
+  dw = inverse<DIM>(Jc, K) * w / J;
+}
+
 // LINE: 3D gradient by the line-per-lane sum factorization (P.sf set,
 // lds = the workgroup's slot buffers, t = thread, slot = this lane's cell).
+// Gu = Grad_xi u (rows: components, columns: xi directions).
 template <int DIM, int N, typename T, bool LINE = false>
 __device__ __forceinline__ void kinematics_prologue(const VmultParams<T> &P, const T *tab, T *A, T *B, T *Xc,
                                                     bool on, bool active, uint32_t code, int r, int i0, int i1, int i2,
-                                                    T (&F)[DIM][DIM], T (&Jri)[DIM][DIM], T &detJr,
+                                                    T (&F)[DIM][DIM], T (&Jri)[DIM][DIM], T &detJr, T (&Gu)[DIM][DIM],
                                                     T *lds = nullptr, int t = 0, int slot = 0) {
   constexpr int NQP = ipow(N, DIM);
   const int node = code & kNodeMask;
@@ -913,7 +953,6 @@ __device__ __forceinline__ void kinematics_prologue(const VmultParams<T> &P, con
       for (int c = 0; c < DIM; ++c) Xc[v * DIM + c] = T(P.coords[(size_t)node * DIM + c]);
   }
   lds_barrier();
-  T Gu[DIM][DIM];
   if constexpr (LINE && DIM == 3)
     SFLine<N, T>::template evaluate<CellShape<DIM, N>::CPW>(P.sf, lds, t, on, slot, r, Gu);
   else
@@ -935,7 +974,7 @@ template <int DIM, int N, typename T>
 __device__ __forceinline__ void kinematics_collocation(const VmultParams<T> &P, const T *tab, T *A, T *Xc, bool on,
                                                        bool active, uint32_t code, int r, int i0, int i1, int i2,
                                                        const T (&uq)[DIM], T (&F)[DIM][DIM], T (&Jri)[DIM][DIM],
-                                                       T &detJr) {
+                                                       T &detJr, T (&Gu)[DIM][DIM]) {
   constexpr int NQP = ipow(N, DIM);
   const int node = code & kNodeMask;
   if (on) {
@@ -952,7 +991,6 @@ __device__ __forceinline__ void kinematics_collocation(const VmultParams<T> &P, 
   const T *Dc = tab + 4 * N * N + 2 * N;
   const int ii[3] = {i0, i1, i2};
   const int stride[3] = {1, N, N * N};
-  T Gu[DIM][DIM];
 #pragma unroll
   for (int c = 0; c < DIM; ++c)
 #pragma unroll
@@ -1095,14 +1133,18 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   }
 
   T F[DIM][DIM], Jri[DIM][DIM], detJr = T(1);
+  T Gu[DIM][DIM], GX[DIM][DIM];  // Grad_xi u, Grad_xi X (curved meshes)
   if constexpr (VAR == V_SCALAR_REF && (PH & PH_QD)) {
     T uq[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; ++c) uq[c] = cch[1 + c];
-    kinematics_collocation<DIM, N, T>(P, tab, A, Xc, on, active, code, r, i0, i1, i2, uq, F, Jri, detJr);
+    // referential: the Q1 metric only, whatever the mesh (:1255-1400)
+    kinematics_collocation<DIM, N, T>(P, tab, A, Xc, on, active, code, r, i0, i1, i2, uq, F, Jri, detJr, Gu);
   } else if constexpr (KIN && (PH & PH_QD)) {
-    kinematics_prologue<DIM, N, T, true>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr, lds, t,
-                                         slot);
+    kinematics_prologue<DIM, N, T, true>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr, Gu,
+                                         lds, t, slot);
+    if (VAR == V_SCALAR && P.curved)
+      geometry_gradient<DIM, N, T, true>(P.coords, P.sf, tab, lds, A, B, on, active, code, t, slot, r, i0, i1, i2, GX);
   }
 
   T G[DIM][DIM];
@@ -1169,9 +1211,12 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
       // quadrature weight from the table (qw after qp)
       const T *qw = tab + 4 * N * N + N;
       const T w = (DIM == 3) ? qw[i0] * qw[i1] * qw[i2] : qw[i0] * qw[i1];
-      const T dw = detJr * w;
+      T dw = detJr * w;
       T K[DIM][DIM];  // J_cur^-1 = J_ref^-1 F^-1
-      grad_times<DIM>(Jri, Fi, K);
+      if (VAR == V_SCALAR && P.curved)
+        eulerian_metric<DIM, T>(GX, Gu, J, w, K, dw);
+      else
+        grad_times<DIM>(Jri, Fi, K);
       if (P.formulation == 0 && VAR != V_ADJOINT) {
         // formulation 0 (include/mf_nh_operator.h:1005-1180): the scalar
         // cache holds J^{-1/d}; the residual evaluates it here
@@ -1419,6 +1464,7 @@ struct CacheParams {
   size_t fs;
   int n_cells;
   int formulation;
+  int curved;  // Eulerian Q_p metric for the spatial caches (tensor2, tensor4)
   int *bad;
 };
 
@@ -1446,8 +1492,13 @@ template <int DIM, int N, typename T, int VAR> __global__ void __launch_bounds__
   KP.formulation = P.formulation;
   KP.u = P.u;
   KP.coords = P.coords;
-  double F[DIM][DIM], Jri[DIM][DIM], detJr;
-  kinematics_prologue<DIM, N, double>(KP, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr);
+  double F[DIM][DIM], Jri[DIM][DIM], detJr, Gu[DIM][DIM], GX[DIM][DIM];
+  kinematics_prologue<DIM, N, double>(KP, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr, Gu);
+  // the spatial caches carry the current metric: Eulerian Q_p on curved meshes
+  constexpr bool SPATIAL = (VAR == V_TENSOR2 || VAR == V_TENSOR4);
+  if (SPATIAL && P.curved)
+    geometry_gradient<DIM, N, double, false>(P.coords, nullptr, tab, lds, A, B, on, active, code, t, slot, r, i0, i1,
+                                             i2, GX);
   double uq[DIM];
   if constexpr (VAR == V_SCALAR_REF) {
     // u at this Gauss point: the tensor-product interpolation of the nodal
@@ -1479,9 +1530,12 @@ template <int DIM, int N, typename T, int VAR> __global__ void __launch_bounds__
   const double cs = m - 2.0 * l * log(J);
   const double *qw = tab + 4 * N * N + N;
   const double w = (DIM == 3) ? qw[i0] * qw[i1] * qw[i2] : qw[i0] * qw[i1];
-  const double dw = detJr * w;
+  double dw = detJr * w;
   double K[DIM][DIM];
-  grad_times<DIM>(Jri, Fi, K);
+  if (SPATIAL && P.curved)
+    eulerian_metric<DIM, double>(GX, Gu, J, w, K, dw);
+  else
+    grad_times<DIM>(Jri, Fi, K);
   T *out = static_cast<T *>(P.cache) + (size_t)cell * NQP + r;
   auto put = [&](int f, double v) { out[(size_t)f * P.fs] = T(v); };
   if constexpr (VAR == V_SCALAR) {
@@ -1595,16 +1649,22 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
     T *A = sf_all + sl * 2 * NB * NQP;
     T *B = A + NB * NQP;
     T *Xc = X_all + sl * NC * DIM;
-    T F[DIM][DIM], Jri[DIM][DIM], detJr;
-    kinematics_prologue<DIM, N, T>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr);
+    T F[DIM][DIM], Jri[DIM][DIM], detJr, Gu[DIM][DIM], GX[DIM][DIM];
+    kinematics_prologue<DIM, N, T>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr, Gu);
+    if (P.curved)
+      geometry_gradient<DIM, N, T, false>(P.coords, P.sf, tab, lds, A, B, on, active, code, t, slot, r, i0, i1, i2,
+                                          GX);
     if (active) {
       const T m = P.mu[cell], l = P.lambda[cell];
       T Fi[DIM][DIM], K[DIM][DIM];
       const T J = inverse<DIM>(F, Fi);
-      grad_times<DIM>(Jri, Fi, K);
       const T *qw = tab + 4 * N * N + N;
       const T w = (DIM == 3) ? qw[i0] * qw[i1] * qw[i2] : qw[i0] * qw[i1];
-      const T dw = detJr * w;
+      T dw = detJr * w;
+      if (P.curved)
+        eulerian_metric<DIM, T>(GX, Gu, J, w, K, dw);
+      else
+        grad_times<DIM>(Jri, Fi, K);
       T Tv[NV], M[NV * (NV + 1) / 2];
       tensor4_fields<DIM, T>(F, J, m, l, dw, P.formulation, Tv, M);
 #pragma unroll

@@ -25,6 +25,7 @@ template <int DIM, int N> struct Launch {
     P.cell_begin = ptr[k];
     P.cell_end = ptr[k + 1];
     P.formulation = op->formulation;
+    P.curved = op->curved;
     P.tnodes = op->d_tile_nodes;
     P.tinfo = op->d_tile_info;
     P.hang = op->d_hang;
@@ -143,6 +144,7 @@ template <int DIM, int N> struct Launch {
     P.fs = op->field_stride;
     P.n_cells = op->n_cells;
     P.formulation = op->formulation;
+    P.curved = op->curved;
     P.bad = op->d_bad;
     hipLaunchKernelGGL((k_cache<DIM, N, T, VAR>), dim3(grid_for(op->n_cells)), dim3(kWG), 0, s, P);
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
@@ -172,6 +174,9 @@ template <int DIM, int N> struct Launch {
     for (int k = 0; k < nc; ++k) {
       VmultParams<T> P = params<T>(op, k);
       P.dst = static_cast<T *>(op->d_diag);
+      // scalar_referential shares the scalar kernel but integrates on the Q1
+      // vertex map on any mesh (include/mf_nh_operator.h:1255-1400)
+      if (op->caching == LSMF_CACHING_SCALAR_REFERENTIAL) P.curved = 0;
       const int n = P.cell_end - P.cell_begin;
       if (n <= 0) continue;
       if (op->tiled)

@@ -127,6 +127,53 @@ bool structured_dims(const lsmf_mesh_desc *m, int dim, int p, int nqp, long long
   return true;
 }
 
+// Some support point lies off its cell's Q1 (multi)linear map of the
+// vertices (beyond round-off relative to the cell size): the current
+// configuration then needs the Eulerian Q_p metric of all support points.
+bool support_points_curved(const lsmf_mesh_desc *m, int dim, int p, int nqp) {
+  const int N = p + 1, NV = 1 << dim;
+  const std::vector<double> g = lobatto_points(p);
+  std::vector<double> wq1((size_t)nqp * NV);  // Q1 weight of vertex v at support point a
+  std::vector<int> corner(NV);
+  for (int v = 0; v < NV; ++v) {
+    int a = 0, stride = 1;
+    for (int d = 0; d < dim; ++d) {
+      a += ((v >> d) & 1) * p * stride;
+      stride *= N;
+    }
+    corner[v] = a;
+  }
+  for (int a = 0; a < nqp; ++a) {
+    int rem = a;
+    double xi[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) {
+      xi[d] = g[rem % N];
+      rem /= N;
+    }
+    for (int v = 0; v < NV; ++v) {
+      double w = 1.0;
+      for (int d = 0; d < dim; ++d) w *= ((v >> d) & 1) ? xi[d] : 1.0 - xi[d];
+      wq1[(size_t)a * NV + v] = w;
+    }
+  }
+  for (int c = 0; c < m->n_cells; ++c) {
+    const int32_t *cn = m->cell_nodes + (size_t)c * nqp;
+    double X[8][3], h = 0.0;
+    for (int v = 0; v < NV; ++v)
+      for (int i = 0; i < dim; ++i) {
+        X[v][i] = m->coords[(size_t)cn[corner[v]] * dim + i];
+        h = std::max(h, std::fabs(X[v][i] - X[0][i]));
+      }
+    for (int a = 0; a < nqp; ++a)
+      for (int i = 0; i < dim; ++i) {
+        double x = 0.0;
+        for (int v = 0; v < NV; ++v) x += wq1[(size_t)a * NV + v] * X[v][i];
+        if (std::fabs(x - m->coords[(size_t)cn[a] * dim + i]) > 1e-10 * h) return true;
+      }
+  }
+  return false;
+}
+
 struct TileDims {
   int tx, ty, tz, nx, ny, nz, tn;
 };
@@ -378,6 +425,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
         min_col[nd] = std::min(min_col[nd], k);
       }
     }
+  op->curved = p > 1 && support_points_curved(mesh, dim, p, nqp);
   // --- cell order of the tiled layout (structured meshes) replaces the
   // colour order; the per-cell index table then carries no first-touch bits
   std::vector<uint32_t> tnodes;

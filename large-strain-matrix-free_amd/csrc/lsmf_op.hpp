@@ -49,6 +49,7 @@ struct lsmf_op {
   int dim = 0, degree = 0, nq = 0, N = 0, nqp = 0;
   int precision = LSMF_FP64, caching = LSMF_CACHING_TENSOR4;
   int formulation = 1;  // material formulation (include/material.h:82-94)
+  bool curved = false;  // some support point off its cell's Q1 map: Eulerian Q_p current metric
   int n_cells = 0, n_nodes = 0;
   size_t n_dofs = 0;
   int n_fields = 0;
