@@ -1803,6 +1803,13 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
   }
 }
 
+// rows of DoFs that no cell touches are zero (the reference zeroes dst and
+// the diagonal before its cell loop, include/mf_nh_operator.h:646-706, 1517)
+template <typename T> __global__ void k_zero_list(const int32_t *__restrict__ list, int n, T *__restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[list[i]] = T(0);
+}
+
 // constrained diagonal entries are one (include/mf_nh_operator.h:1533-1534)
 template <typename T> __global__ void k_set_one(const int32_t *__restrict__ list, int n_cons, T *__restrict__ diag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -58,6 +58,12 @@ template <int DIM, int N> struct Launch {
     if (op->n_node_cons > 0 && !add)
       hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((op->n_dofs + 255) / 256)), dim3(256), 0, s, dst, op->n_dofs);
   }
+  // DoFs that no cell touches keep whatever dst held: zero them (set mode)
+  template <typename T> static void zero_orphans(const lsmf_op *op, T *dst, hipStream_t s) {
+    if (op->n_orphan > 0)
+      hipLaunchKernelGGL(k_zero_list<T>, dim3((op->n_orphan + 255) / 256), dim3(256), 0, s, op->d_orphan,
+                         op->n_orphan, dst);
+  }
   static int n_colours(const lsmf_op *op) {
     return (int)(op->tiled ? op->tile_ptr.size() : op->colour_ptr.size()) - 1;
   }
@@ -89,6 +95,7 @@ template <int DIM, int N> struct Launch {
     const T *src = static_cast<const T *>(src_);
     constexpr unsigned FULL = PH_SF | PH_QD;
     zero_if_constrained(op, dst, add, s);
+    if (!add) zero_orphans(op, dst, s);
     if (constr) {
       if (ph == FULL) {
         if (add) run_colours<T, VAR, true, true, FULL>(op, dst, src, s);
@@ -185,6 +192,7 @@ template <int DIM, int N> struct Launch {
         hipLaunchKernelGGL((k_diagonal<DIM, N, T, VAR, false>), dim3(grid_for(n)), dim3(kWG), 0, s, P);
     }
     T *diag = static_cast<T *>(op->d_diag);
+    zero_orphans(op, diag, s);
     if (op->n_cons > 0)
       hipLaunchKernelGGL(k_set_one<T>, dim3((op->n_cons + 255) / 256), dim3(256), 0, s, op->d_cons, op->n_cons, diag);
     const size_t n = op->n_dofs;
@@ -238,6 +246,7 @@ template <int DIM, int N> struct Launch {
     if (op->precision != LSMF_FP64) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual needs an FP64 operator");
     constexpr unsigned FULL = PH_SF | PH_QD;
     zero_if_constrained(op, rhs, false, s);
+    zero_orphans(op, rhs, s);
     run_colours<double, V_RESIDUAL, false, false, FULL>(op, rhs, nullptr, s);
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
@@ -246,6 +255,7 @@ template <int DIM, int N> struct Launch {
     if (op->precision != LSMF_FP64) return fail(LSMF_ERR_INVALID_ARGUMENT, "residual VJP needs an FP64 operator");
     constexpr unsigned FULL = PH_SF | PH_QD;
     zero_if_constrained(op, du, false, s);
+    zero_orphans(op, du, s);
     run_colours<double, V_ADJOINT, false, false, FULL>(op, du, w, s);
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }

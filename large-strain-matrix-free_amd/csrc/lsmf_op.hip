@@ -79,7 +79,7 @@ template <typename T> int dalloc(T **p, size_t n, const char *what) {
 void free_op(lsmf_op *op) {
   void *ptrs[] = {op->d_idx,  op->d_coords,    op->d_tab,  op->d_tab64, op->d_mu64,     op->d_lambda64,
                   op->d_mu,   op->d_lambda,    op->d_cache, op->d_diag, op->d_inv_diag, op->d_cons,
-                  op->d_bad,  op->d_perm,      op->d_tile_nodes, op->d_tile_info, op->d_hang, op->d_hptr,
+                  op->d_bad,  op->d_perm,      op->d_tile_nodes, op->d_tile_info, op->d_orphan, op->d_hang, op->d_hptr,
                   op->d_hmaster, op->d_hw, op->d_node_colour, op->d_fixed, op->d_probe};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -461,6 +461,17 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   for (int k = 0; k < nnc; ++k)
     for (int c = 0; c < dim; ++c) cons.push_back(mesh->nc_nodes[k] * dim + c);
   op->n_cons = (int)cons.size();
+  // DoFs of nodes that no cell references: their rows are zero, and the
+  // first-touch stores never write them
+  std::vector<int32_t> orphan;
+  {
+    std::vector<uint8_t> used(mesh->n_nodes, 0);
+    for (size_t i = 0; i < (size_t)nc * nqp; ++i) used[mesh->cell_nodes[i]] = 1;
+    for (int nd = 0; nd < mesh->n_nodes; ++nd)
+      if (!used[nd])
+        for (int c = 0; c < dim; ++c) orphan.push_back(nd * dim + c);
+  }
+  op->n_orphan = (int)orphan.size();
 
   // --- node-constraint tables and the distance-2 node colouring that the
   // probing diagonal uses (nodes written by a common cell differ in colour)
@@ -545,6 +556,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       (rc = dalloc((char **)&op->d_diag, op->n_dofs * es, "diagonal")) ||
       (rc = dalloc((char **)&op->d_inv_diag, op->n_dofs * es, "inverse diagonal")) ||
       (rc = dalloc(&op->d_cons, cons.size(), "constrained list")) || (rc = dalloc(&op->d_bad, 1, "counter")) ||
+      (rc = dalloc(&op->d_orphan, orphan.size(), "unreferenced DoFs")) ||
       (rc = dalloc(&op->d_perm, (size_t)nc, "cell permutation")) ||
       (rc = dalloc(&op->d_tile_nodes, tnodes.size(), "tile nodes")) ||
       (rc = dalloc(&op->d_tile_info, tinfo.size(), "tile info")) ||
@@ -582,6 +594,8 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   }
   if (e == hipSuccess && !cons.empty())
     e = hipMemcpy(op->d_cons, cons.data(), cons.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !orphan.empty())
+    e = hipMemcpy(op->d_orphan, orphan.data(), orphan.size() * sizeof(int32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(op->d_perm, op->perm.data(), (size_t)nc * sizeof(int32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess && !tnodes.empty())
     e = hipMemcpy(op->d_tile_nodes, tnodes.data(), tnodes.size() * sizeof(uint32_t), hipMemcpyHostToDevice);

@@ -72,6 +72,8 @@ struct lsmf_op {
   void *d_diag = nullptr, *d_inv_diag = nullptr;
   int32_t *d_cons = nullptr;        // constrained DoF list
   int n_cons = 0;
+  int32_t *d_orphan = nullptr;      // DoFs of nodes that no cell references (zero rows)
+  int n_orphan = 0;
   int *d_bad = nullptr;
   int32_t *d_perm = nullptr;        // sorted position -> mesh cell
   double *adj_dmu = nullptr, *adj_dlambda = nullptr;  // outputs of the running residual VJP
