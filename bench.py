@@ -208,6 +208,9 @@ def main():
                     help="coarse-level solver of the V-cycle (chebyshev = the reference's choice)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the caching-variant sweep (32^3 Q2) and the adjoint time/memory trade (48^3 Q2)")
+    ap.add_argument("--solve-only", default=None, choices=["fp32", "fp64"],
+                    help="run only the Newton solve with this level precision and print its statistics "
+                         "as one JSON line (no vmult timing; for kernel traces of the solve)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, run 3 plain device copies of the source vector (known bytes) so "
                          "HBM counters (FETCH_SIZE / WRITE_SIZE) can be scaled")
@@ -227,6 +230,18 @@ def main():
     caching = args.caching or cfg["caching"]
     progress(f"building config {args.config} mesh")
     mesh, mu, lam = lsmf.build_config_mesh(args.config, n=args.n)
+    if args.solve_only:
+        if "load_steps" not in cfg:
+            print(f"config {args.config} has no Newton solve", file=sys.stderr)
+            sys.exit(2)
+        progress(f"Newton solve with {args.solve_only} GMG levels (tol {args.solve_tol:g})")
+        r = newton_solve(args.config, args.n, mu, lam, args.solve_only, args.solve_tol, args.coarse)
+        st = r["stats"]
+        print(json.dumps({"config": args.config, "level_precision": args.solve_only, "mg_levels": r["n_levels"],
+                          "solve_s": round(sum(x["time_total_s"] for x in st), 3),
+                          "newton_iterations": [x["newton_iterations"] for x in st],
+                          "cg_iterations": [x["linear_iterations"] for x in st]}))
+        return
     u, _ = lsmf.smooth_displacement(mesh.coords, 0.02, lsmf.workload.SEED + args.config)
     cons = mesh.constrained_mask({0: list(range(mesh.dim)), 1: [0]})
     dev = torch.device("cuda:0")
