@@ -94,6 +94,35 @@ template <int DIM, int VAR> struct CacheFields {
                                                      : 0;
 };
 
+// Position of cache field f at point cq = sorted cell * NQP + q.  FP64 fields
+// are separate arrays of fs values; FP32 fields are stored in pairs (one
+// float2 per point and pair of fields, 2 fs floats per pair), so that a lane
+// loads two fields with one 8-byte access, as many bytes per load
+// instruction as the FP64 cache.
+template <typename T> __host__ __device__ constexpr size_t cache_at(int f, size_t cq, size_t fs) {
+  return sizeof(T) == 8 ? (size_t)f * fs + cq : (size_t)(f >> 1) * 2 * fs + 2 * cq + (size_t)(f & 1);
+}
+// cache size in values for nf fields (FP32: whole pairs)
+template <typename T> __host__ __device__ constexpr size_t cache_values(int nf, size_t fs) {
+  return sizeof(T) == 8 ? (size_t)nf * fs : (size_t)((nf + 1) / 2) * 2 * fs;
+}
+// load the nf fields of point cq (two per 8-byte load for FP32)
+template <typename T, int NF> __device__ __forceinline__ void load_cache(const T *__restrict__ c, size_t cq, size_t fs,
+                                                                         T (&v)[NF]) {
+  if constexpr (sizeof(T) == 8) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) v[f] = c[(size_t)f * fs + cq];
+  } else {
+    const float2 *c2 = reinterpret_cast<const float2 *>(c) + cq;
+#pragma unroll
+    for (int f = 0; f < NF; f += 2) {
+      const float2 p = c2[(size_t)(f >> 1) * fs];
+      v[f] = p.x;
+      if (f + 1 < NF) v[f + 1] = p.y;
+    }
+  }
+}
+
 // 1D tables: S, D, St, Dt (N*N each), Gauss points and weights (N each),
 // then the collocation derivative on the Gauss points (N*N, used by
 // scalar_referential only)
@@ -1107,11 +1136,7 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   const int r_s = on ? r : 0;
   __builtin_amdgcn_sched_barrier(0);
   T cch[NF > 0 ? NF : 1];
-  if constexpr ((PH & PH_QD) && NF > 0 && !RECOMP) {
-    const T *cp = P.cache + cell_s * NQP + r_s;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) cch[f] = cp[(size_t)f * P.fs];
-  }
+  if constexpr ((PH & PH_QD) && NF > 0 && !RECOMP) load_cache<T, NF>(P.cache, cell_s * NQP + r_s, P.fs, cch);
   __builtin_amdgcn_sched_barrier(0);
 
   // tile-local index of this lane's node (tiled layout)
@@ -1536,8 +1561,9 @@ template <int DIM, int N, typename T, int VAR> __global__ void __launch_bounds__
     eulerian_metric<DIM, double>(GX, Gu, J, w, K, dw);
   else
     grad_times<DIM>(Jri, Fi, K);
-  T *out = static_cast<T *>(P.cache) + (size_t)cell * NQP + r;
-  auto put = [&](int f, double v) { out[(size_t)f * P.fs] = T(v); };
+  T *out = static_cast<T *>(P.cache);
+  const size_t cq = (size_t)cell * NQP + r;
+  auto put = [&](int f, double v) { out[cache_at<T>(f, cq, P.fs)] = T(v); };
   if constexpr (VAR == V_SCALAR) {
     // formulation 0 caches J^{-1/d} (include/mf_nh_operator.h:467-487)
     put(0, P.formulation == 0 ? pow(J, -1.0 / DIM) : cs);
@@ -1678,9 +1704,10 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
     }
   } else {
     if (active) {
-      const T *cp = P.cache + (size_t)cell * NQP + r;
+      T v[NF];
+      load_cache<T, NF>(P.cache, (size_t)cell * NQP + r, P.fs, v);
 #pragma unroll
-      for (int f = 0; f < NF; ++f) qd[f * NQP + r] = cp[(size_t)f * P.fs];
+      for (int f = 0; f < NF; ++f) qd[f * NQP + r] = v[f];
     }
   }
   lds_barrier();
