@@ -94,32 +94,43 @@ template <int DIM, int VAR> struct CacheFields {
                                                      : 0;
 };
 
-// Position of cache field f at point cq = sorted cell * NQP + q.  FP64 fields
-// are separate arrays of fs values; FP32 fields are stored in pairs (one
-// float2 per point and pair of fields, 2 fs floats per pair), so that a lane
-// loads two fields with one 8-byte access, as many bytes per load
-// instruction as the FP64 cache.
-template <typename T> __host__ __device__ constexpr size_t cache_at(int f, size_t cq, size_t fs) {
-  return sizeof(T) == 8 ? (size_t)f * fs + cq : (size_t)(f >> 1) * 2 * fs + 2 * cq + (size_t)(f & 1);
+// Position of cache field f (of nf) at point cq = sorted cell * NQP + q.
+// The fields are stored in groups of G = 16 / sizeof(T) (two FP64 or four
+// FP32 fields: one 16-byte vector per point and group, G * fs values per
+// group), so that a lane loads G fields with one 16-byte access; the
+// nf % G fields left over form one narrower group after them (no padding).
+template <typename T> constexpr int kCacheGroup = 16 / (int)sizeof(T);
+template <typename T> __host__ __device__ constexpr size_t cache_at(int f, int nf, size_t cq, size_t fs) {
+  constexpr int G = kCacheGroup<T>;
+  const int full = nf / G * G;
+  return f < full ? (size_t)(f / G) * G * fs + (size_t)G * cq + (size_t)(f % G)
+                  : (size_t)full * fs + (size_t)(nf - full) * cq + (size_t)(f - full);
 }
-// cache size in values for nf fields (FP32: whole pairs)
-template <typename T> __host__ __device__ constexpr size_t cache_values(int nf, size_t fs) {
-  return sizeof(T) == 8 ? (size_t)nf * fs : (size_t)((nf + 1) / 2) * 2 * fs;
-}
-// load the nf fields of point cq (two per 8-byte load for FP32)
+template <typename T> struct CacheVec;
+template <> struct CacheVec<double> { using type = double2; };
+template <> struct CacheVec<float> { using type = float4; };
+// load the NF fields of point cq: G per 16-byte load, the rest by the
+// widest access that fits (FP32: a pair as one 8-byte load)
 template <typename T, int NF> __device__ __forceinline__ void load_cache(const T *__restrict__ c, size_t cq, size_t fs,
                                                                          T (&v)[NF]) {
-  if constexpr (sizeof(T) == 8) {
+  constexpr int G = kCacheGroup<T>, FULL = NF / G * G, TAIL = NF - FULL;
+  using V = typename CacheVec<T>::type;
+  const V *cv = reinterpret_cast<const V *>(c) + cq;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) v[f] = c[(size_t)f * fs + cq];
+  for (int f = 0; f < FULL; f += G) {
+    const V p = cv[(size_t)(f / G) * fs];
+    const T *e = reinterpret_cast<const T *>(&p);
+#pragma unroll
+    for (int k = 0; k < G; ++k) v[f + k] = e[k];
+  }
+  const T *ct = c + (size_t)FULL * fs + (size_t)TAIL * cq;
+  if constexpr (TAIL == 2 && sizeof(T) == 4) {
+    const float2 p = *reinterpret_cast<const float2 *>(ct);
+    v[FULL] = p.x;
+    v[FULL + 1] = p.y;
   } else {
-    const float2 *c2 = reinterpret_cast<const float2 *>(c) + cq;
 #pragma unroll
-    for (int f = 0; f < NF; f += 2) {
-      const float2 p = c2[(size_t)(f >> 1) * fs];
-      v[f] = p.x;
-      if (f + 1 < NF) v[f + 1] = p.y;
-    }
+    for (int k = 0; k < TAIL; ++k) v[FULL + k] = ct[k];
   }
 }
 
@@ -1563,7 +1574,8 @@ template <int DIM, int N, typename T, int VAR> __global__ void __launch_bounds__
     grad_times<DIM>(Jri, Fi, K);
   T *out = static_cast<T *>(P.cache);
   const size_t cq = (size_t)cell * NQP + r;
-  auto put = [&](int f, double v) { out[cache_at<T>(f, cq, P.fs)] = T(v); };
+  constexpr int NFC = CacheFields<DIM, VAR>::value;
+  auto put = [&](int f, double v) { out[cache_at<T>(f, NFC, cq, P.fs)] = T(v); };
   if constexpr (VAR == V_SCALAR) {
     // formulation 0 caches J^{-1/d} (include/mf_nh_operator.h:467-487)
     put(0, P.formulation == 0 ? pow(J, -1.0 / DIM) : cs);

@@ -552,10 +552,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       (rc = dalloc((char **)&op->d_mu, (size_t)nc * es, "mu")) ||
       (rc = dalloc((char **)&op->d_lambda, (size_t)nc * es, "lambda")) ||
       (rc = dalloc(&op->d_u64, op->n_dofs, "displacement")) ||
-      (rc = dalloc((char **)&op->d_cache,
-                   (op->precision == LSMF_FP64 ? cache_values<double>(op->n_fields, op->field_stride)
-                                               : cache_values<float>(op->n_fields, op->field_stride)) * es,
-                   "cache")) ||
+      (rc = dalloc((char **)&op->d_cache, (size_t)op->n_fields * op->field_stride * es, "cache")) ||
       (rc = dalloc((char **)&op->d_diag, op->n_dofs * es, "diagonal")) ||
       (rc = dalloc((char **)&op->d_inv_diag, op->n_dofs * es, "inverse diagonal")) ||
       (rc = dalloc(&op->d_cons, cons.size(), "constrained list")) || (rc = dalloc(&op->d_bad, 1, "counter")) ||
@@ -581,8 +578,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
     return rc;
   }
   op->bytes_idx = idx.size() * sizeof(uint32_t);
-  op->bytes_cache = (op->precision == LSMF_FP64 ? cache_values<double>(op->n_fields, op->field_stride)
-                                                : cache_values<float>(op->n_fields, op->field_stride)) * es;
+  op->bytes_cache = (size_t)op->n_fields * op->field_stride * es;
   hipError_t e = hipSuccess;
   e = hipMemcpy(op->d_idx, idx.data(), idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess)
