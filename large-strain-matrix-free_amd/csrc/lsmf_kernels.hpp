@@ -164,6 +164,25 @@ template <int DIM, int N> struct TileShape {
 };
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;  // tile node outside the mesh
 
+// The DIM values of one node as one access (3D FP32: 12 bytes in one load,
+// 3D FP64: 16 + 8 bytes), instead of DIM separate loads each touching three
+// times the cache lines per instruction.
+template <int DIM, typename T> struct NodeVec {
+  T v[DIM];
+};
+template <int DIM, typename T> __device__ __forceinline__ void load_node(const T *p, T (&v)[DIM]) {
+  NodeVec<DIM, T> n;
+  __builtin_memcpy(&n, p, sizeof(n));
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) v[c] = n.v[c];
+}
+template <int DIM, typename T> __device__ __forceinline__ void store_node(T *p, const T (&v)[DIM]) {
+  NodeVec<DIM, T> n;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) n.v[c] = v[c];
+  __builtin_memcpy(p, &n, sizeof(n));
+}
+
 // LDS-only workgroup barrier: waits for this wave's LDS traffic, not for its
 // outstanding global loads, so the per-qp cache loads issued at kernel entry
 // stay in flight across the sum-factorization stages.
@@ -1123,16 +1142,11 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
     tcode = P.tnodes[(size_t)tile * TN + (t < TN ? t : 0)];
     tvalid = t < TN && tcode != kNoNode;
     tnode = tvalid ? (tcode & kNodeMask) : 0;
-    if constexpr (VAR != V_RESIDUAL) {
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) sv[c] = P.src[tnode * DIM + c];
-    }
+    if constexpr (VAR != V_RESIDUAL) load_node<DIM, T>(P.src + tnode * DIM, sv);
     // the node's destination values, read now rather than at the final
     // read-modify-write: no other tile of this colour touches the node, so
     // they do not change before this tile writes them
-    if (tvalid && (ADD || !(tcode & kFirstBit)))
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) dv[c] = P.dst[tnode * DIM + c];
+    if (tvalid && (ADD || !(tcode & kFirstBit))) load_node<DIM, T>(P.dst + tnode * DIM, dv);
   } else {
     cell = P.cell_begin + L * CPW + slot;
     active = on && (cell < P.cell_end);
@@ -1400,10 +1414,15 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
 #pragma unroll
             for (int c = 0; c < DIM; ++c) acc[c] += res[lane * DIM + c];
           }
+      T nv[DIM];
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        if (CONSTR && ((tcode >> (kConsShift + c)) & 1u)) continue;
-        P.dst[tnode * DIM + c] = dv[c] + acc[c];  // dv = 0 on first touch
+      for (int c = 0; c < DIM; ++c) nv[c] = dv[c] + acc[c];  // dv = 0 on first touch
+      if (!CONSTR || !((tcode >> kConsShift) & ((1u << DIM) - 1u))) {
+        store_node<DIM, T>(P.dst + tnode * DIM, nv);
+      } else {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+          if (!((tcode >> (kConsShift + c)) & 1u)) P.dst[tnode * DIM + c] = nv[c];
       }
     }
     return;
