@@ -148,6 +148,14 @@ lsmf_status lsmf_op_destroy(lsmf_op *op);
  * (include/material.h:82-94). */
 lsmf_status lsmf_set_material_per_cell(lsmf_op *op, const double *mu, const double *lambda);
 
+/* The reference's two-material form of set_material(matrix, inclusion)
+ * (include/mf_nh_operator.h:596-608, selection at :930-932): cell c takes
+ * the inclusion's (mu1, lambda1) where material_id[c] != 0 and the matrix's
+ * (mu0, lambda0) elsewhere; material_id is a host array [n_cells] in the
+ * mesh's cell order. */
+lsmf_status lsmf_set_material_two(lsmf_op *op, const uint8_t *material_id, double mu0, double lambda0, double mu1,
+                                  double lambda1);
+
 /* The formulation argument of Material_Compressible_Neo_Hook_One_Field
  * (include/material.h:82-94), which the reference's set_material passes in:
  * 1 (default): psi = mu/2 (tr C - d - 2 ln J) + lambda ln^2 J;

@@ -660,6 +660,17 @@ lsmf_status lsmf_set_material_per_cell(lsmf_op *op, const double *mu, const doub
   return LSMF_OK;
 }
 
+lsmf_status lsmf_set_material_two(lsmf_op *op, const uint8_t *material_id, double mu0, double lambda0, double mu1,
+                                  double lambda1) {
+  if (!op || !material_id) return fail(LSMF_ERR_INVALID_ARGUMENT, "set_material: null argument");
+  std::vector<double> m(op->n_cells), l(op->n_cells);
+  for (int c = 0; c < op->n_cells; ++c) {
+    m[c] = material_id[c] ? mu1 : mu0;
+    l[c] = material_id[c] ? lambda1 : lambda0;
+  }
+  return lsmf_set_material_per_cell(op, m.data(), l.data());
+}
+
 lsmf_status lsmf_set_formulation(lsmf_op *op, int formulation) {
   if (!op) return fail(LSMF_ERR_INVALID_ARGUMENT, "set_formulation: null operator");
   if (formulation != 0 && formulation != 1)

@@ -92,6 +92,10 @@ template <typename number> class NeoHookOperatorHIP {
   void set_material(const double *mu, const double *lambda) {
     abi_check(lsmf_set_material_per_cell(op_, mu, lambda), "set_material");
   }
+  // the reference's set_material(matrix, inclusion): material_id != 0 selects the inclusion
+  void set_material(const uint8_t *material_id, double mu0, double lambda0, double mu1, double lambda1) {
+    abi_check(lsmf_set_material_two(op_, material_id, mu0, lambda0, mu1, lambda1), "set_material");
+  }
   void set_linearization(const double *u_dev) { abi_check(lsmf_set_linearization(op_, u_dev, stream_), "set_linearization"); }
   void cache() { abi_check(lsmf_cache(op_, stream_), "cache"); }
   void compute_diagonal() { abi_check(lsmf_compute_diagonal(op_, stream_), "compute_diagonal"); }

@@ -87,6 +87,7 @@ _sig = {
     "lsmf_op_create": (_i, [ctypes.POINTER(_MeshDesc), ctypes.POINTER(_OpOptions), ctypes.POINTER(_vp)]),
     "lsmf_op_destroy": (_i, [_vp]),
     "lsmf_set_material_per_cell": (_i, [_vp, _vp, _vp]),
+    "lsmf_set_material_two": (_i, [_vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "lsmf_set_linearization": (_i, [_vp, _vp, _vp]),
     "lsmf_set_formulation": (_i, [_vp, _i]),
     "lsmf_cache": (_i, [_vp, _vp]),
@@ -253,6 +254,14 @@ class NeoHookOperator:
         lam = np.ascontiguousarray(lam, dtype=np.float64)
         assert mu.shape == (self.n_cells,) and lam.shape == (self.n_cells,)
         _check(_lib.lsmf_set_material_per_cell(self._h, mu.ctypes.data, lam.ctypes.data))
+
+    def set_material_two(self, material_id, matrix, inclusion):
+        """the reference's set_material(matrix, inclusion): (mu, lambda) of
+        the inclusion where material_id != 0, of the matrix elsewhere"""
+        mid = np.ascontiguousarray(material_id, dtype=np.uint8)
+        assert mid.shape == (self.n_cells,)
+        _check(_lib.lsmf_set_material_two(self._h, mid.ctypes.data, float(matrix[0]), float(matrix[1]),
+                                          float(inclusion[0]), float(inclusion[1])))
 
     def set_linearization(self, u, stream=None):
         import torch

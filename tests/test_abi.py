@@ -105,6 +105,7 @@ def test_argument_errors_before_device_work():
     assert lsmf._lib.lsmf_op_create(ctypes.byref(desc_col), ctypes.byref(ok), ctypes.byref(h)) == -1
     assert b"share a node" in lsmf._lib.lsmf_last_error()
     assert lsmf._lib.lsmf_vmult(None, None, None, 0, 31, None) == -1
+    assert lsmf._lib.lsmf_set_material_two(None, None, 1.0, 1.0, 2.0, 2.0) == -1
 
 
 def test_workload_generators_are_seeded():

@@ -317,3 +317,28 @@ def test_tiled_layout_equals_cell_layout(dim, p, n, device):
         op.vmult(y, v.float())
         op32[layout] = y.cpu().double().numpy()
     assert rel(op32["auto"], op32["cells"]) < 1e-5
+
+
+def test_two_material_setter_equals_per_cell_arrays(device):
+    """lsmf_set_material_two (the reference's set_material(matrix,
+    inclusion), material_id != 0 -> inclusion) gives the same operator as
+    the equivalent per-cell arrays, bit for bit"""
+    import torch
+    mesh, _, _, u, cons = problem(3, 2, 4)
+    mid = (mesh.cell_centres()[:, 0] > 0.5).astype(np.uint8)
+    m0, l0, m1, l1 = 0.4225, 0.5769, 42.25, 57.69
+    ud = torch.tensor(u, device=device)
+    v = torch.tensor(np.random.default_rng(3).standard_normal(mesh.n_dofs), device=device)
+    out = []
+    for two in (True, False):
+        op = lsmf.NeoHookOperator(mesh, caching="tensor4", constrained=cons)
+        if two:
+            op.set_material_two(mid, (m0, l0), (m1, l1))
+        else:
+            op.set_material(np.where(mid, m1, m0), np.where(mid, l1, l0))
+        op.set_linearization(ud)
+        op.cache()
+        y = torch.empty_like(v)
+        op.vmult(y, v)
+        out.append(y.cpu().numpy())
+    assert np.array_equal(out[0], out[1])
