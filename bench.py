@@ -80,7 +80,7 @@ def cpu_baseline(cfg_id, caching, n_sample, seconds):
         "unit": "GDoF/s",
         "cores": oracle.num_threads(),
         "kind": "port",
-        "sample": (f"{n_sample}^3-cell Q2 {caching} FP64 vmult of the same configuration "
+        "sample": (f"{n_sample}^3-cell Q{mesh.degree} {caching} FP64 vmult of the same configuration "
                    f"({mesh.n_dofs} DoF), {n} vmults in {el:.1f} s, oracle C++ (OpenMP, sum factorization)"),
     }
 
@@ -157,16 +157,18 @@ def variant_sweep(cfg_id, reps):
     return out
 
 
-def adjoint_trade(cfg_id, reps):
+def adjoint_trade(cfg_id, reps, caching="tensor4"):
     """residual VJP that recomputes the kinematics (no stored state) against
-    the tensor4 cache a cached backward would keep: time and bytes"""
+    the cache a cached backward would keep (the tangent cache of `caching`
+    plus b - I and ln J per point): time and bytes"""
     import torch
     import lsmf
     mesh, mu, lam = lsmf.build_config_mesh(cfg_id)
     u, _ = lsmf.smooth_displacement(mesh.coords, 0.01, lsmf.workload.SEED + cfg_id)
     dev = torch.device("cuda:0")
     ud = torch.tensor(u, device=dev)
-    op = lsmf.NeoHookOperator(mesh, precision="fp64", caching="tensor4")
+    del u
+    op = lsmf.NeoHookOperator(mesh, precision="fp64", caching=caching)
     op.set_material(mu, lam)
     op.set_linearization(ud)
     w = torch.rand(mesh.n_dofs, dtype=torch.float64, device=dev)
@@ -178,13 +180,18 @@ def adjoint_trade(cfg_id, reps):
     t_cache = time_ms(lambda: op.cache(), max(1, reps // 4), torch)
     t_vmult = time_ms(lambda: op.vmult(r, w, mask="plain"), reps, torch)
     n_qp = mesh.n_cells * (mesh.degree + 1) ** mesh.dim
+    nv = mesh.dim * (mesh.dim + 1) // 2
+    n_fields = {"tensor4": mesh.dim ** 2 + nv + nv * (nv + 1) // 2, "tensor2": mesh.dim ** 2 + nv + 2}[caching]
     out = {"workload": f"config {cfg_id}: {mesh.n_cells_dir[0]}^{mesh.dim} Q{mesh.degree} log-normal mu/lambda, "
                        f"{mesh.n_dofs} DoF, FP64",
            "residual_ms": round(t_res, 4), "vjp_recompute_ms": round(t_vjp, 4),
-           "tensor4_cache_build_ms": round(t_cache, 4), "tensor4_vmult_ms": round(t_vmult, 4),
-           "vjp_state_bytes": 0, "tensor4_cache_bytes": 36 * 8 * n_qp,
-           "note": "VJP w.r.t. u, mu, lambda recomputes F, J, ln J per point; the cached alternative "
-                   "(tensor4 cache + per-point b - I and ln J) would hold 36 + 7 FP64 values per point"}
+           "cached_caching": caching,
+           "cache_build_ms": round(t_cache, 4), "cached_vmult_ms": round(t_vmult, 4),
+           "vjp_state_bytes": 0, "cache_bytes": n_fields * 8 * n_qp,
+           "cached_backward_bytes": (n_fields + nv + 1) * 8 * n_qp,
+           "note": f"VJP w.r.t. u, mu, lambda recomputes F, J, ln J per point; the cached alternative "
+                   f"({caching} cache + per-point b - I and ln J) would hold {n_fields} + {nv + 1} FP64 values "
+                   f"per point and apply the cached tangent for the u part"}
     del op
     torch.cuda.empty_cache()
     return out
@@ -331,8 +338,12 @@ def main():
         progress("caching-variant sweep on config 1 (32^3 Q2)")
         variants = variant_sweep(1, 20)
         progress(f"  GDoF/s {variants['gdofs']}")
-        progress("adjoint time/memory trade on config 3 (48^3 Q2)")
-        adjoint = adjoint_trade(3, 20)
+        if args.config == 4:
+            progress("adjoint time/memory trade on config 4 (64^3 Q4, tensor2)")
+            adjoint = adjoint_trade(4, 10, "tensor2")
+        else:
+            progress("adjoint time/memory trade on config 3 (48^3 Q2)")
+            adjoint = adjoint_trade(3, 20)
         progress(f"  {adjoint}")
 
     cpu = None

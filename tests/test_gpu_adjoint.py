@@ -4,11 +4,13 @@
     VJP matches the reference's tangent vmult; JVP = VJP by symmetry
   * central finite differences (h = 1e-6) on 64 seeded directions in
     (u, mu, lambda) on the 48^3 Q2 configuration with log-normal per-cell
-    mu, lambda (sigma = 0.5): relative error <= 1e-6
+    mu, lambda (sigma = 0.5): relative error <= 1e-6; the same on 16
+    directions for the Q4 element of the 64^3 Q4 configuration
   * the torch.autograd.Function gives the same gradient as differences of
     a scalar loss
   * gradient of a face-misfit objective through a Newton solve (one adjoint
-    GMG-CG solve + one VJP) against differences of two full solves
+    GMG-CG solve + one VJP) against differences of two full solves, also
+    with Q4 tensor2 and FP32 multigrid levels
 """
 import numpy as np
 import pytest
@@ -92,6 +94,44 @@ def test_vjp_matches_finite_differences_config3(device):
     log(f"  worst relative FD error {worst:.2e}")
 
 
+def test_vjp_matches_finite_differences_q4(device):
+    """config 4's element (Q4, 5^3 Gauss points, log-normal sigma = 0.3) on
+    a 4^3 mesh: central differences on 16 seeded directions"""
+    import torch
+    log("[vjp FD] 4^3 Q4, log-normal mu/lambda, 16 directions")
+    mesh = lsmf.Mesh(3, 4, [4, 4, 4])
+    mu, lam = lsmf.lognormal_material(mesh.n_cells, 1.0, 0.45, 0.3, 41)
+    u, _ = lsmf.smooth_displacement(mesh.coords, 0.02, 43)
+    op = make_op(mesh, mu, lam, u, caching="tensor2")
+    rng = np.random.default_rng(44)
+    w = torch.tensor(rng.standard_normal(mesh.n_dofs), device=device)
+    du = torch.empty_like(w)
+    dmu = torch.empty(mesh.n_cells, dtype=torch.float64, device=device)
+    dla = torch.empty_like(dmu)
+    op.residual_vjp(w, du, dmu, dla)
+    du, dmu, dla = du.cpu().numpy(), dmu.cpu().numpy(), dla.cpu().numpy()
+    ud = torch.tensor(u, device=device)
+    r = torch.empty_like(ud)
+    h = 1e-6
+    worst = 0.0
+    for k in range(16):
+        a = rng.standard_normal(mesh.n_dofs) * 0.01
+        b = rng.standard_normal(mesh.n_cells) * np.asarray(mu)
+        c = rng.standard_normal(mesh.n_cells) * np.asarray(lam)
+        vals = []
+        for sgn in (1.0, -1.0):
+            op.set_material(mu + sgn * h * b, lam + sgn * h * c)
+            op.set_linearization(ud + sgn * h * torch.tensor(a, device=device))
+            op.residual(r)
+            vals.append(float(w @ r))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        an = float(du @ a + dmu @ b + dla @ c)
+        err = abs(fd - an) / abs(an)
+        worst = max(worst, err)
+        assert err < 1e-6, (k, fd, an)
+    log(f"  worst relative FD error {worst:.2e}")
+
+
 def test_autograd_function(device):
     import torch
     mesh = lsmf.Mesh(3, 2, [4, 4, 4])
@@ -121,17 +161,22 @@ def test_autograd_function(device):
     assert abs(fd - an) < 1e-6 * abs(an), (fd, an)
 
 
-def test_misfit_gradient_through_newton_solve(device):
+@pytest.mark.parametrize("degree,caching,mg_precision,sigma", [(2, "tensor4", "fp64", 0.5),
+                                                                (4, "tensor2", "fp32", 0.3)])
+def test_misfit_gradient_through_newton_solve(degree, caching, mg_precision, sigma, device):
     """x=0 clamped, u_x = 0.01 at x=1 (one load step); J = misfit of the x=1
     face displacement (free components) against the solution for a second
-    seeded log-normal field; dJ/dmu, dJ/dlambda vs differences of solves"""
-    log("[adjoint solve] 8^3 Q2")
-    nc, L = [2, 2, 2], 3
-    mesh = lsmf.Mesh(3, 2, [k << (L - 1) for k in nc])
+    seeded log-normal field; dJ/dmu, dJ/dlambda vs differences of solves.
+    Q2 with FP64 levels on 8^3 cells; config 4's combination (Q4, tensor2,
+    FP32 V-cycle inside the FP64 CG) on 4^3 cells."""
+    nc = [2, 2, 2]
+    L = 3 if degree == 2 else 2
+    log(f"[adjoint solve] {nc[0] << (L - 1)}^3 Q{degree} {caching}, {mg_precision} levels")
+    mesh = lsmf.Mesh(3, degree, [k << (L - 1) for k in nc])
     cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
-    o = lsmf.solver_options(dim=3, degree=2, n_q_1d=3, n_cells_coarse=nc, n_levels=L, caching="tensor4",
-                            mg_caching="tensor4", mg_precision="fp64", coarse_solver="direct", tol_lin=1e-12,
-                            tol_f=1e-13, tol_u=1e-12, dt=1.0, t_end=1.0,
+    o = lsmf.solver_options(dim=3, degree=degree, n_q_1d=degree + 1, n_cells_coarse=nc, n_levels=L,
+                            caching=caching, mg_caching=caching, mg_precision=mg_precision,
+                            coarse_solver="direct", tol_lin=1e-12, tol_f=1e-13, tol_u=1e-12, dt=1.0, t_end=1.0,
                             dirichlet={0: ([0, 1, 2], [0.0] * 3), 1: ([0], [0.01])})
 
     def solve(mu, lam):
@@ -139,9 +184,9 @@ def test_misfit_gradient_through_newton_solve(device):
         S.run()
         return S, S.solution()
 
-    mu_t, lam_t = lsmf.lognormal_material(mesh.n_cells, 1.0, 0.3, 0.5, 31)
+    mu_t, lam_t = lsmf.lognormal_material(mesh.n_cells, 1.0, 0.3, sigma, 31)
     _, target = solve(mu_t, lam_t)
-    mu, lam = lsmf.lognormal_material(mesh.n_cells, 1.0, 0.3, 0.5, 32)
+    mu, lam = lsmf.lognormal_material(mesh.n_cells, 1.0, 0.3, sigma, 32)
     S, u_star = solve(mu, lam)
     face = mesh.boundary_nodes(1)
     J, gmu, gla = misfit_gradient(S, mesh, u_star, mu, lam, target, face, cons)
