@@ -189,6 +189,14 @@ lsmf_status lsmf_compute_diagonal(lsmf_op *op, lsmf_stream stream);
  * src must not alias. */
 lsmf_status lsmf_vmult(lsmf_op *op, void *dst, const void *src, int add, unsigned mask, lsmf_stream stream);
 
+/* n_vectors products dst_b = A src_b of lsmf_vmult (set mode, constrained
+ * rows as identity), src_b = src + b * n and dst_b = dst + b * n with n =
+ * lsmf_op_size(op), in one launch per colour for all vectors; each dst_b is
+ * bit-identical to lsmf_vmult's.  No counterpart in the reference, whose
+ * coarse level is solved iteratively: it serves the exact coarse solve,
+ * which probes the coarse matrix with the n unit vectors at once. */
+lsmf_status lsmf_vmult_batched(lsmf_op *op, void *dst, const void *src, int n_vectors, lsmf_stream stream);
+
 /* get_matrix_diagonal_inverse() / el(i,i) (include/mf_nh_operator.h:206-211,
  * 1561-1571): device pointers to the diagonal and its inverse (operator
  * precision), valid until the next compute_diagonal or destroy. */

@@ -928,6 +928,7 @@ template <typename Number> struct VmultParams {
   const Number *__restrict__ tab;     // S, D, St, Dt, qp, qw (Number)
   const Number *__restrict__ src;
   Number *__restrict__ dst;
+  size_t vstride;                     // batched products: distance between vectors (0: one vector)
   const Number *__restrict__ u;       // scalar / residual: displacement
   const double *__restrict__ coords;  // scalar / residual: node coordinates
   const Number *__restrict__ mu;      // per sorted cell
@@ -1101,6 +1102,10 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
   T *Xall = tab + TABN;
   T *Rall = Xall + (KIN ? CPW * NC * DIM : 0);
 
+  // batched products (all columns of a small matrix in one launch): vector
+  // blockIdx.y lives at offset blockIdx.y * vstride of src and dst
+  const T *__restrict__ src = P.src + (size_t)blockIdx.y * P.vstride;
+  T *__restrict__ dst = P.dst + (size_t)blockIdx.y * P.vstride;
   const int t = threadIdx.x;
   if (t < TABN) tab[t] = P.tab[t];
   const int slot = t / NQP;
@@ -1142,11 +1147,11 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
     tcode = P.tnodes[(size_t)tile * TN + (t < TN ? t : 0)];
     tvalid = t < TN && tcode != kNoNode;
     tnode = tvalid ? (tcode & kNodeMask) : 0;
-    if constexpr (VAR != V_RESIDUAL) load_node<DIM, T>(P.src + tnode * DIM, sv);
+    if constexpr (VAR != V_RESIDUAL) load_node<DIM, T>(src + tnode * DIM, sv);
     // the node's destination values, read now rather than at the final
     // read-modify-write: no other tile of this colour touches the node, so
     // they do not change before this tile writes them
-    if (tvalid && (ADD || !(tcode & kFirstBit))) load_node<DIM, T>(P.dst + tnode * DIM, dv);
+    if (tvalid && (ADD || !(tcode & kFirstBit))) load_node<DIM, T>(dst + tnode * DIM, dv);
   } else {
     cell = P.cell_begin + L * CPW + slot;
     active = on && (cell < P.cell_end);
@@ -1155,7 +1160,7 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
     node = code & kNodeMask;
     if constexpr (!KIN) {
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) sv[c] = P.src[node * DIM + c];
+      for (int c = 0; c < DIM; ++c) sv[c] = src[node * DIM + c];
     }
   }
   const int r_s = on ? r : 0;
@@ -1216,10 +1221,10 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
           for (int j = P.hptr[hidx]; j < P.hptr[hidx + 1]; ++j) {
             const uint32_t mc = P.hmaster[j];
             if (CONSTR && ((mc >> (kConsShift + c)) & 1u)) continue;
-            v = fma(P.hw[j], P.src[(size_t)(mc & kNodeMask) * DIM + c], v);
+            v = fma(P.hw[j], src[(size_t)(mc & kNodeMask) * DIM + c], v);
           }
         } else {
-          if constexpr (KIN) v = P.src[node * DIM + c];
+          if constexpr (KIN) v = src[node * DIM + c];
           else v = sv[c];
           if (CONSTR && ((code >> (kConsShift + c)) & 1u)) v = T(0);
         }
@@ -1418,11 +1423,11 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) nv[c] = dv[c] + acc[c];  // dv = 0 on first touch
       if (!CONSTR || !((tcode >> kConsShift) & ((1u << DIM) - 1u))) {
-        store_node<DIM, T>(P.dst + tnode * DIM, nv);
+        store_node<DIM, T>(dst + tnode * DIM, nv);
       } else {
 #pragma unroll
         for (int c = 0; c < DIM; ++c)
-          if (!((tcode >> (kConsShift + c)) & 1u)) P.dst[tnode * DIM + c] = nv[c];
+          if (!((tcode >> (kConsShift + c)) & 1u)) dst[tnode * DIM + c] = nv[c];
       }
     }
     return;
@@ -1436,12 +1441,12 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
       const bool skip = CONSTR && ((code >> (kConsShift + c)) & 1u);
-      old[c] = (first || skip) ? T(0) : P.dst[node * DIM + c];
+      old[c] = (first || skip) ? T(0) : dst[node * DIM + c];
     }
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
       if (CONSTR && ((code >> (kConsShift + c)) & 1u)) continue;
-      P.dst[node * DIM + c] = old[c] + out[c];
+      dst[node * DIM + c] = old[c] + out[c];
     }
   }
   if (P.hang) {
@@ -1465,7 +1470,7 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
 #pragma unroll
           for (int c = 0; c < DIM; ++c) {
             if (CONSTR && ((mc >> (kConsShift + c)) & 1u)) continue;
-            atomicAdd(P.dst + m * DIM + c, P.hw[j] * hres[(slot * NQP + rr) * DIM + c]);
+            atomicAdd(dst + m * DIM + c, P.hw[j] * hres[(slot * NQP + rr) * DIM + c]);
           }
         }
       }
@@ -1863,6 +1868,22 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
 
 // rows of DoFs that no cell touches are zero (the reference zeroes dst and
 // the diagonal before its cell loop, include/mf_nh_operator.h:646-706, 1517)
+// constrained rows of a batch of vectors (vector blockIdx.y at blockIdx.y * stride)
+template <typename T>
+__global__ void k_constrained_identity_batch(const int32_t *__restrict__ list, int n, const T *__restrict__ src,
+                                             T *__restrict__ dst, size_t stride) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const size_t k = (size_t)blockIdx.y * stride + (size_t)list[i];
+    dst[k] = src[k];
+  }
+}
+template <typename T>
+__global__ void k_zero_list_batch(const int32_t *__restrict__ list, int n, T *__restrict__ dst, size_t stride) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[(size_t)blockIdx.y * stride + (size_t)list[i]] = T(0);
+}
+
 template <typename T> __global__ void k_zero_list(const int32_t *__restrict__ list, int n, T *__restrict__ dst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[list[i]] = T(0);

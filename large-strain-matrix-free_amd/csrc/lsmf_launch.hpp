@@ -115,6 +115,61 @@ template <int DIM, int N> struct Launch {
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
 
+  // nvec products in set mode with constraints, vector b at b * n_dofs:
+  // one launch per colour for all vectors (grid y = vector), the same code
+  // path per vector as vmult_t
+  template <typename T, int VAR> static int vmult_batched_t(const lsmf_op *op, T *dst, const T *src, int nvec,
+                                                            hipStream_t s) {
+    constexpr unsigned FULL = PH_SF | PH_QD;
+    constexpr int kMaxY = 65535;
+    const size_t n = op->n_dofs;
+    const int nc = n_colours(op);
+    for (int v0 = 0; v0 < nvec; v0 += kMaxY) {
+      const unsigned nv = (unsigned)std::min(nvec - v0, kMaxY);
+      T *d = dst + (size_t)v0 * n;
+      const T *x = src + (size_t)v0 * n;
+      if (op->n_node_cons > 0)
+        hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((nv * n + 255) / 256)), dim3(256), 0, s, d, nv * n);
+      if (op->n_orphan > 0)
+        hipLaunchKernelGGL(k_zero_list_batch<T>, dim3((op->n_orphan + 255) / 256, nv), dim3(256), 0, s,
+                           op->d_orphan, op->n_orphan, d, n);
+      for (int k = 0; k < nc; ++k) {
+        VmultParams<T> P = params<T>(op, k);
+        P.src = x;
+        P.dst = d;
+        P.vstride = n;
+        const int nb = P.cell_end - P.cell_begin;
+        if (nb <= 0) continue;
+        if (op->tiled)
+          hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, false, true, FULL, true>), dim3(nb, nv), dim3(kWG), 0, s, P);
+        else
+          hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, false, true, FULL, false>), dim3(grid_for(nb), nv), dim3(kWG),
+                             0, s, P);
+      }
+      if (op->n_cons > 0)
+        hipLaunchKernelGGL(k_constrained_identity_batch<T>, dim3((op->n_cons + 255) / 256, nv), dim3(256), 0, s,
+                           op->d_cons, op->n_cons, x, d, n);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
+  }
+
+  static int vmult_batched(const lsmf_op *op, void *dst, const void *src, int nvec, hipStream_t s) {
+    const bool f64 = op->precision == LSMF_FP64;
+#define LSMF_BATCH(V)                                                                          \
+  return f64 ? vmult_batched_t<double, V>(op, (double *)dst, (const double *)src, nvec, s) \
+             : vmult_batched_t<float, V>(op, (float *)dst, (const float *)src, nvec, s)
+    switch (op->caching) {
+      case LSMF_CACHING_TENSOR4: LSMF_BATCH(V_TENSOR4);
+      case LSMF_CACHING_TENSOR2: LSMF_BATCH(V_TENSOR2);
+      case LSMF_CACHING_SCALAR: LSMF_BATCH(V_SCALAR);
+      case LSMF_CACHING_TENSOR4_NS: LSMF_BATCH(V_TENSOR4_NS);
+      case LSMF_CACHING_SCALAR_REFERENTIAL: LSMF_BATCH(V_SCALAR_REF);
+      default:
+        return fail(LSMF_ERR_UNKNOWN_CACHING, "vmult_batched: caching variant not available on the GPU");
+    }
+#undef LSMF_BATCH
+  }
+
   static int vmult(const lsmf_op *op, void *dst, const void *src, bool add, bool constr, unsigned ph,
                    hipStream_t s) {
     const bool f64 = op->precision == LSMF_FP64;
@@ -261,7 +316,7 @@ template <int DIM, int N> struct Launch {
   }
 
   static const LaunchTable *table() {
-    static const LaunchTable t{&vmult, &cache, &diagonal, &residual, &residual_vjp};
+    static const LaunchTable t{&vmult, &vmult_batched, &cache, &diagonal, &residual, &residual_vjp};
     return &t;
   }
 };

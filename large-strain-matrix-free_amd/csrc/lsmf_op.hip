@@ -742,6 +742,18 @@ lsmf_status lsmf_vmult(lsmf_op *op, void *dst, const void *src, int add, unsigne
   return LSMF_OK;
 }
 
+lsmf_status lsmf_vmult_batched(lsmf_op *op, void *dst, const void *src, int n_vectors, lsmf_stream stream) {
+  if (!op || !dst || !src) return fail(LSMF_ERR_INVALID_ARGUMENT, "vmult_batched: null argument");
+  if (n_vectors <= 0) return fail(LSMF_ERR_INVALID_ARGUMENT, "vmult_batched: n_vectors must be positive");
+  const size_t bytes = (size_t)n_vectors * op->n_dofs * (op->precision == LSMF_FP64 ? 8 : 4);
+  const char *d = static_cast<const char *>(dst), *x = static_cast<const char *>(src);
+  if (d < x + bytes && x < d + bytes) return fail(LSMF_ERR_INVALID_ARGUMENT, "vmult_batched: dst and src overlap");
+  if (!op->cached) return fail(LSMF_ERR_NOT_CACHED, "vmult_batched: call cache() first");
+  const int rc = launch_table(op->dim, op->N)->vmult_batched(op, dst, src, n_vectors, (hipStream_t)stream);
+  if (rc) return fail(rc, "vmult_batched: kernel launch failed");
+  return LSMF_OK;
+}
+
 lsmf_status lsmf_get_diagonal(lsmf_op *op, const void **diag_dev, const void **inv_diag_dev) {
   if (!op) return fail(LSMF_ERR_INVALID_ARGUMENT, "get_diagonal: null operator");
   if (!op->diag_ok) return fail(LSMF_ERR_NOT_CACHED, "get_diagonal: call compute_diagonal() first");

@@ -105,6 +105,7 @@ size_t elem_size(const lsmf_op *op);
 // launch entry points of one (dim, N) instantiation unit
 struct LaunchTable {
   int (*vmult)(const lsmf_op *, void *dst, const void *src, bool add, bool constr, unsigned ph, hipStream_t);
+  int (*vmult_batched)(const lsmf_op *, void *dst, const void *src, int nvec, hipStream_t);
   int (*cache)(const lsmf_op *, hipStream_t);
   int (*diagonal)(const lsmf_op *, hipStream_t);
   int (*residual)(const lsmf_op *, double *rhs, hipStream_t);

@@ -464,8 +464,9 @@ template <typename T> class GMG : public GMGBase {
     graph_r_ = nullptr;
   }
 
-  // FP64 coarse matrix by probing unit vectors (column j = A e_j, stored as
-  // row j: A is symmetric), symmetrised and inverted in place on the device
+  // FP64 coarse matrix by probing with the n unit vectors in one batched
+  // product (column j = A e_j, stored as row j: A is symmetric),
+  // symmetrised and inverted in place on the device
   void assemble_coarse_inverse(const double *u0) {
     Level<T> &v = lv_[0];
     const int n = (int)v.g->n_dofs;
@@ -473,15 +474,12 @@ template <typename T> class GMG : public GMGBase {
       coarse64_->set_linearization(u0);
       coarse64_->cache();
     }
-    DeviceArray<double> e(n), piv(2 * (size_t)n);
+    const size_t nn = (size_t)n * n;
+    DeviceArray<double> e(nn), piv(2 * (size_t)n);
     DeviceArray<int> bad(1);
     double *A = ainv_.p;
-    for (int j = 0; j < n; ++j) {
-      hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, s_, e.p, 0.0, (size_t)n);
-      hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(1), 0, s_, e.p + j, 1.0, (size_t)1);
-      coarse64_->vmult(A + (size_t)j * n, e.p);
-    }
-    const size_t nn = (size_t)n * n;
+    hipLaunchKernelGGL(k_identity, dim3(nblk(nn)), dim3(256), 0, s_, e.p, n);
+    coarse64_->vmult_batched(A, e.p, n);
     hipLaunchKernelGGL(k_sym_average, dim3(nblk(nn)), dim3(256), 0, s_, A, n);
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(1), 0, s_, bad.p, 0, (size_t)1);
     for (int k = 0; k < n; ++k) {

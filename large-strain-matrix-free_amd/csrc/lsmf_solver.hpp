@@ -102,6 +102,10 @@ template <typename number> class NeoHookOperatorHIP {
   void vmult(number *dst, const number *src, unsigned mask = LSMF_MASK_DEFAULT) const {
     abi_check(lsmf_vmult(op_, dst, src, 0, mask, stream_), "vmult");
   }
+  // n_vectors products at once (vector b at b * m()): the coarse-matrix probing
+  void vmult_batched(number *dst, const number *src, int n_vectors) const {
+    abi_check(lsmf_vmult_batched(op_, dst, src, n_vectors, stream_), "vmult_batched");
+  }
   void vmult_add(number *dst, const number *src) const { abi_check(lsmf_vmult(op_, dst, src, 1, LSMF_MASK_DEFAULT, stream_), "vmult_add"); }
   void Tvmult(number *dst, const number *src) const { vmult(dst, src); }
   void Tvmult_add(number *dst, const number *src) const { vmult_add(dst, src); }

@@ -202,6 +202,13 @@ __global__ void k_dense_matvec(const double *__restrict__ M, const T *__restrict
 // saves row k and column k as they are before step k (and flags a pivot that
 // is not positive), k_gj_step applies step k to every entry. SPD keeps every
 // pivot positive, so no row exchanges are needed.
+// the n x n identity, row j = e_j (the probe vectors of the coarse matrix)
+__global__ void k_identity(double *__restrict__ E, int n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * n) return;
+  E[t] = (t / n == t % n) ? 1.0 : 0.0;
+}
+
 __global__ void k_sym_average(double *__restrict__ A, int n) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * n) return;

@@ -93,6 +93,7 @@ _sig = {
     "lsmf_cache": (_i, [_vp, _vp]),
     "lsmf_compute_diagonal": (_i, [_vp, _vp]),
     "lsmf_vmult": (_i, [_vp, _vp, _vp, _i, ctypes.c_uint, _vp]),
+    "lsmf_vmult_batched": (_i, [_vp, _vp, _vp, _i, _vp]),
     "lsmf_get_diagonal": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "lsmf_precondition_jacobi": (_i, [_vp, _vp, _vp, ctypes.c_double, _vp]),
     "lsmf_op_size": (ctypes.c_size_t, [_vp]),
@@ -282,6 +283,14 @@ class NeoHookOperator:
 
     def vmult_add(self, dst, src, stream=None):
         self.vmult(dst, src, add=True, stream=stream)
+
+    def vmult_batched(self, dst, src, stream=None):
+        """dst[b] = A src[b] for every row b of two contiguous (k, m()) tensors,
+        in one launch per colour"""
+        assert dst.dtype == self.dtype and src.dtype == self.dtype
+        assert dst.dim() == 2 and dst.shape == src.shape and dst.shape[1] == self.m()
+        assert dst.is_contiguous() and src.is_contiguous()
+        _check(_lib.lsmf_vmult_batched(self._h, _ptr(dst), _ptr(src), int(dst.shape[0]), _stream(stream)))
 
     Tvmult = vmult
 

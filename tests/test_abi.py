@@ -106,6 +106,7 @@ def test_argument_errors_before_device_work():
     assert b"share a node" in lsmf._lib.lsmf_last_error()
     assert lsmf._lib.lsmf_vmult(None, None, None, 0, 31, None) == -1
     assert lsmf._lib.lsmf_set_material_two(None, None, 1.0, 1.0, 2.0, 2.0) == -1
+    assert lsmf._lib.lsmf_vmult_batched(None, None, None, 1, None) == -1
 
 
 def test_workload_generators_are_seeded():

@@ -72,6 +72,14 @@ def test_hanging_operator_matches_oracle(dim, p, caching, device):
         assert rel(pl.cpu().double().numpy(), ref.vmult(vv, plain=True)) < tol, prec
         op.compute_diagonal()
         assert rel(op.diagonal().cpu().double().numpy(), ref.diagonal()) < tol, prec
+        # batched products (zeroed first, masters added) equal the single ones
+        X = torch.stack([src, acc, pl])
+        Y = torch.full_like(X, np.nan)
+        op.vmult_batched(Y, X)
+        for b in range(3):
+            y = torch.full_like(src, np.nan)
+            op.vmult(y, X[b].contiguous())
+            assert torch.equal(Y[b], y), (prec, b)
     r = torch.full((mesh.n_dofs,), np.nan, dtype=torch.float64, device=device)
     op = lsmf.NeoHookOperator(mesh, precision="fp64", caching=caching, constrained=cons, use_mesh_colouring=False,
                               node_constraints=nc)

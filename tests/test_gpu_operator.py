@@ -342,3 +342,36 @@ def test_two_material_setter_equals_per_cell_arrays(device):
         op.vmult(y, v)
         out.append(y.cpu().numpy())
     assert np.array_equal(out[0], out[1])
+
+
+BATCH_CASES = [(3, 2, 3, "auto", "tensor4", "fp64"), (3, 2, 3, "cells", "tensor4", "fp64"),
+               (3, 4, 2, "auto", "tensor2", "fp64"), (2, 2, 5, "auto", "scalar", "fp32"),
+               (3, 1, 4, "auto", "tensor4_ns", "fp32"), (2, 4, 3, "cells", "scalar_referential", "fp64")]
+
+
+@pytest.mark.parametrize("dim,p,n,layout,caching,precision", BATCH_CASES,
+                         ids=["-".join(map(str, c)) for c in BATCH_CASES])
+def test_batched_vmult_equals_single_products(dim, p, n, layout, caching, precision, device):
+    """lsmf_vmult_batched: every row of the batch bit-identical to
+    lsmf_vmult of that row (constrained rows as identity); the unit vectors
+    of the batch probe the columns of the operator"""
+    import torch
+    log(f"[batched] dim={dim} p={p} n={n} {layout} {caching} {precision}")
+    mesh, mu, lam, u, cons = problem(dim, p, n)
+    op = lsmf.NeoHookOperator(mesh, precision=precision, caching=caching, constrained=cons, layout=layout)
+    op.set_material(mu, lam)
+    op.set_linearization(torch.tensor(u, device=device))
+    op.cache()
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    k = 37
+    X = torch.tensor(np.random.default_rng(8).standard_normal((k, mesh.n_dofs)), dtype=dt, device=device)
+    X[5].zero_()
+    X[6].zero_()
+    X[6, 3 * dim + 1] = 1.0  # a unit vector
+    Y = torch.full_like(X, np.nan)
+    op.vmult_batched(Y, X)
+    for b in range(k):
+        y = torch.full((mesh.n_dofs,), np.nan, dtype=dt, device=device)
+        op.vmult(y, X[b].contiguous())
+        assert torch.equal(Y[b], y), b
+    assert float(Y[5].abs().max()) == 0.0
