@@ -466,7 +466,7 @@ template <typename T> class GMG : public GMGBase {
 
   // FP64 coarse matrix by probing with the n unit vectors in one batched
   // product (column j = A e_j, stored as row j: A is symmetric),
-  // symmetrised and inverted in place on the device
+  // symmetrised and inverted in place on the device (blocked Gauss-Jordan)
   void assemble_coarse_inverse(const double *u0) {
     Level<T> &v = lv_[0];
     const int n = (int)v.g->n_dofs;
@@ -475,18 +475,22 @@ template <typename T> class GMG : public GMGBase {
       coarse64_->cache();
     }
     const size_t nn = (size_t)n * n;
-    DeviceArray<double> e(nn), piv(2 * (size_t)n);
+    DeviceArray<double> e(nn), W((size_t)kGJB * kGJB), Cold((size_t)n * kGJB), Cnew((size_t)n * kGJB),
+        R((size_t)kGJB * n);
     DeviceArray<int> bad(1);
     double *A = ainv_.p;
     hipLaunchKernelGGL(k_identity, dim3(nblk(nn)), dim3(256), 0, s_, e.p, n);
     coarse64_->vmult_batched(A, e.p, n);
     hipLaunchKernelGGL(k_sym_average, dim3(nblk(nn)), dim3(256), 0, s_, A, n);
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(1), 0, s_, bad.p, 0, (size_t)1);
-    for (int k = 0; k < n; ++k) {
-      hipLaunchKernelGGL(k_gj_pivot, dim3(nblk(n)), dim3(256), 0, s_, (const double *)A, n, k, piv.p, piv.p + n,
-                         bad.p);
-      hipLaunchKernelGGL(k_gj_step, dim3(nblk(nn)), dim3(256), 0, s_, A, n, k, (const double *)piv.p,
-                         (const double *)(piv.p + n));
+    const dim3 tiles((unsigned)((n + 63) / 64), (unsigned)((n + 15) / 16));
+    for (int K0 = 0; K0 < n; K0 += kGJB) {
+      const int bk = std::min(kGJB, n - K0);
+      hipLaunchKernelGGL(k_bgj_diag, dim3(1), dim3(256), 0, s_, (const double *)A, n, K0, bk, W.p, bad.p);
+      hipLaunchKernelGGL(k_bgj_panels, dim3(nblk(2 * (size_t)n * kGJB)), dim3(256), 0, s_, (const double *)A, n, K0,
+                         bk, (const double *)W.p, Cold.p, Cnew.p, R.p);
+      hipLaunchKernelGGL(k_bgj_update, tiles, dim3(256), 0, s_, A, n, K0, bk, (const double *)W.p,
+                         (const double *)Cold.p, (const double *)Cnew.p, (const double *)R.p);
     }
     int bad_h = 0;
     hip_check(hipMemcpyAsync(&bad_h, bad.p, sizeof(int), hipMemcpyDeviceToHost, s_), "coarse inverse");

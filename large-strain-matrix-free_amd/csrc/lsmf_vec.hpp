@@ -197,11 +197,6 @@ __global__ void k_dense_matvec(const double *__restrict__ M, const T *__restrict
   if (lane == 0) x[row] = T(s);
 }
 
-// In-place inverse of a dense SPD matrix (row-major, n x n) by Gauss-Jordan
-// elimination without pivoting, one pivot per pair of launches: k_gj_pivot
-// saves row k and column k as they are before step k (and flags a pivot that
-// is not positive), k_gj_step applies step k to every entry. SPD keeps every
-// pivot positive, so no row exchanges are needed.
 // the n x n identity, row j = e_j (the probe vectors of the coarse matrix)
 __global__ void k_identity(double *__restrict__ E, int n) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -219,29 +214,116 @@ __global__ void k_sym_average(double *__restrict__ A, int n) {
   A[(size_t)j * n + i] = a;
 }
 
-__global__ void k_gj_pivot(const double *__restrict__ A, int n, int k, double *__restrict__ rowk,
-                           double *__restrict__ colk, int *__restrict__ bad) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  rowk[i] = A[(size_t)k * n + i];
-  colk[i] = A[(size_t)i * n + k];
-  if (i == k && !(A[(size_t)k * n + k] > 0.0)) bad[0] = 1;
+// In-place inverse of a dense SPD matrix (row-major, n x n) by blocked
+// Gauss-Jordan elimination without pivoting, kGJB rows and columns per block
+// step K = [K0, K0 + bk), three launches per step:
+//   k_bgj_diag    W = inverse of the diagonal block A_KK (unblocked
+//                 Gauss-Jordan in LDS; flags a pivot that is not positive)
+//   k_bgj_panels  Cold = A_:K, Cnew = -A_:K W (rows outside K), R = W A_K:
+//   k_bgj_update  A_KK = W, A_Kj = R_j, A_iK = Cnew_i, A_ij -= Cold_i . R_j
+// The Schur complements of an SPD matrix are SPD, so every pivot stays
+// positive and no row exchanges are needed.  Scratch: W kGJB^2, Cold and
+// Cnew n * kGJB (row i at i * kGJB), R kGJB * n (row r at r * n).
+constexpr int kGJB = 32;
+
+__global__ void __launch_bounds__(256) k_bgj_diag(const double *__restrict__ A, int n, int K0, int bk,
+                                                  double *__restrict__ W, int *__restrict__ bad) {
+  __shared__ double M[kGJB * kGJB];
+  __shared__ double rk[kGJB], ck[kGJB];
+  const int t = threadIdx.x;
+  for (int e = t; e < kGJB * kGJB; e += 256) {
+    const int i = e / kGJB, j = e % kGJB;
+    M[e] = (i < bk && j < bk) ? A[(size_t)(K0 + i) * n + K0 + j] : 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < bk; ++k) {
+    if (t < bk) {
+      rk[t] = M[k * kGJB + t];
+      ck[t] = M[t * kGJB + k];
+    }
+    __syncthreads();
+    const double p = rk[k];
+    if (t == 0 && !(p > 0.0)) bad[0] = 1;
+    const double pinv = 1.0 / p;
+    for (int e = t; e < bk * kGJB; e += 256) {
+      const int i = e / kGJB, j = e % kGJB;
+      if (j >= bk) continue;
+      double v;
+      if (i == k)
+        v = j == k ? pinv : rk[j] * pinv;
+      else if (j == k)
+        v = -ck[i] * pinv;
+      else
+        v = M[e] - ck[i] * rk[j] * pinv;
+      M[e] = v;
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < kGJB * kGJB; e += 256) W[e] = M[e];
 }
 
-__global__ void k_gj_step(double *__restrict__ A, int n, int k, const double *__restrict__ rowk,
-                          const double *__restrict__ colk) {
+__global__ void k_bgj_panels(const double *__restrict__ A, int n, int K0, int bk, const double *__restrict__ W,
+                             double *__restrict__ Cold, double *__restrict__ Cnew, double *__restrict__ R) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)n * n) return;
-  const int i = (int)(t / n), j = (int)(t % n);
-  const double pinv = 1.0 / rowk[k];
-  double v;
-  if (i == k)
-    v = j == k ? pinv : rowk[j] * pinv;
-  else if (j == k)
-    v = -colk[i] * pinv;
-  else
-    v = A[t] - colk[i] * rowk[j] * pinv;
-  A[t] = v;
+  const size_t half = (size_t)n * kGJB;
+  if (t < half) {
+    const int i = (int)(t / kGJB), c = (int)(t % kGJB);
+    if (c >= bk) return;
+    const double *a = A + (size_t)i * n + K0;
+    Cold[t] = a[c];
+    if (i >= K0 && i < K0 + bk) return;
+    double s = 0.0;
+    for (int m = 0; m < bk; ++m) s = fma(a[m], W[m * kGJB + c], s);
+    Cnew[t] = -s;
+  } else if (t < 2 * half) {
+    const size_t u = t - half;
+    const int r = (int)(u / n), j = (int)(u % n);
+    if (r >= bk) return;
+    double s = 0.0;
+    for (int m = 0; m < bk; ++m) s = fma(W[r * kGJB + m], A[(size_t)(K0 + m) * n + j], s);
+    R[(size_t)r * n + j] = s;
+  }
+}
+
+// one workgroup per 16 x 64 tile of A: lane -> column, wave -> 4 rows; the
+// tile's Cold rows and R columns are staged in LDS
+__global__ void __launch_bounds__(256) k_bgj_update(double *__restrict__ A, int n, int K0, int bk,
+                                                    const double *__restrict__ W, const double *__restrict__ Cold,
+                                                    const double *__restrict__ Cnew, const double *__restrict__ R) {
+  __shared__ double Rs[kGJB * 64];
+  __shared__ double Cs[16 * kGJB];
+  const int t = threadIdx.x;
+  const int j0 = blockIdx.x * 64, i0 = blockIdx.y * 16;
+  for (int e = t; e < kGJB * 64; e += 256) {
+    const int m = e / 64, jj = e % 64;
+    Rs[e] = (m < bk && j0 + jj < n) ? R[(size_t)m * n + j0 + jj] : 0.0;
+  }
+  for (int e = t; e < 16 * kGJB; e += 256) {
+    const int ii = e / kGJB, m = e % kGJB;
+    Cs[e] = (m < bk && i0 + ii < n) ? Cold[(size_t)(i0 + ii) * kGJB + m] : 0.0;
+  }
+  __syncthreads();
+  const int jj = t % 64, w = t / 64;
+  const int j = j0 + jj;
+  if (j >= n) return;
+  const bool jK = j >= K0 && j < K0 + bk;
+  for (int q = 0; q < 4; ++q) {
+    const int ii = w * 4 + q, i = i0 + ii;
+    if (i >= n) break;
+    const bool iK = i >= K0 && i < K0 + bk;
+    double *a = A + (size_t)i * n + j;
+    if (iK && jK) {
+      *a = W[(i - K0) * kGJB + (j - K0)];
+    } else if (iK) {
+      *a = Rs[(i - K0) * 64 + jj];
+    } else if (jK) {
+      *a = Cnew[(size_t)i * kGJB + (j - K0)];
+    } else {
+      double v = *a;
+      for (int m = 0; m < bk; ++m) v = fma(-Cs[ii * kGJB + m], Rs[m * 64 + jj], v);
+      *a = v;
+    }
+  }
 }
 
 // One banded 1D operator applied along direction `dir` of a node array of
