@@ -327,6 +327,8 @@ def main():
             "cg_iterations": {p: [s["linear_iterations"] for s in res[p]["stats"]] for p in res},
             "final_rel_residual": {p: [s["residual_final"] / s["residual_0"] for s in res[p]["stats"]] for p in res},
             "setup_s": {p: round(sum(s["time_setup_s"] for s in res[p]["stats"]), 3) for p in res},
+            "setup_caches_diagonals_s": {p: round(sum(s["time_setup_cache_s"] for s in res[p]["stats"]), 3)
+                                         for p in res},
             "linear_s": {p: round(sum(s["time_linear_s"] for s in res[p]["stats"]), 3) for p in res},
             "rel_diff_u_fp32_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
             "timed": "Newton loop of all load steps (residual, caches, level setup, CG); hierarchy creation excluded",

@@ -336,6 +336,8 @@ typedef struct {
   double residual_0, residual_final;
   double time_total_s, time_setup_s, time_linear_s, time_residual_s;
   double cond_number;     /* last estimate of the step (0 when not estimated) */
+  double time_setup_cache_s; /* of time_setup_s: caches and diagonals of the fine operator and
+                                all levels (the rest: Chebyshev eigenvalue estimates, coarse solve) */
 } lsmf_step_stats;
 
 typedef struct lsmf_solver lsmf_solver;

@@ -231,6 +231,8 @@ static void tridiag_extremes(const std::vector<double> &a, const std::vector<dou
 struct GMGBase {
   virtual ~GMGBase() {}
   virtual void setup(const double *u_fine, double *tmp64a, double *tmp64b) = 0;
+  // wall time of the last setup's caches and diagonals (all levels)
+  virtual double setup_cache_seconds() const = 0;
   virtual void precondition(double *z, const double *r) = 0;
   virtual void info(int level, double &lmax, double &lmin, int &degree) const = 0;
   virtual int n_levels() const = 0;
@@ -338,6 +340,7 @@ template <typename T> class GMG : public GMGBase {
   }
   ~GMG() override { drop_graph(); }
 
+  double setup_cache_seconds() const override { return cache_s_; }
   int n_levels() const override { return (int)lv_.size(); }
   size_t level_dofs(int l) const override { return lv_[l].g->n_dofs; }
   void info(int l, double &lmax, double &lmin, int &degree) const override {
@@ -385,6 +388,9 @@ template <typename T> class GMG : public GMGBase {
   void setup(const double *u_fine, double *ta, double *tb) override {
     drop_graph();
     const int L = (int)lv_.size();
+    // caches and diagonals of every level first (timed on their own), then
+    // the Chebyshev data and the coarse solve
+    const auto t0 = Clock::now();
     const double *ucur = u_fine;
     for (int l = L - 1; l >= 0; --l) {
       Level<T> &v = lv_[l];
@@ -399,8 +405,13 @@ template <typename T> class GMG : public GMGBase {
         v.op->cache();
       }
       v.op->compute_diagonal();
+    }
+    hip_check(hipStreamSynchronize(s_), "level caches");
+    cache_s_ = seconds_since(t0);
+    for (int l = L - 1; l >= 0; --l) {
+      Level<T> &v = lv_[l];
       if (l == 0 && opt_.coarse_solver == LSMF_COARSE_DIRECT) {
-        assemble_coarse_inverse(ucur);
+        assemble_coarse_inverse(l < L - 1 ? lv_[0].u64.p : u_fine);
       } else if (l == 0) {
         const int its = opt_.coarse_eig_iterations > 0 ? opt_.coarse_eig_iterations : (int)v.g->n_dofs;
         estimate_eigenvalues(v, its);
@@ -599,6 +610,7 @@ template <typename T> class GMG : public GMGBase {
   Reducer red_;
   std::vector<Level<T>> lv_;
   size_t tmax64_ = 0;
+  double cache_s_ = 0.0;
   NeoHookOperatorHIP<double> *coarse64_ = nullptr;
   std::unique_ptr<NeoHookOperatorHIP<double>> coarse64_own_;
   DeviceArray<double> ainv_;
@@ -1075,6 +1087,7 @@ static void newton_step(lsmf_solver *S, double t, int step, lsmf_step_stats &st)
     S->fine.set_linearization(S->u_tot.p);
     S->fine.cache();
     st.time_setup_s += seconds_since(t0);
+    st.time_setup_cache_s += seconds_since(t0);
     t0 = Clock::now();
     // assemble_system: residual + Neumann + lifting (-K g) at it 0
     abi_check(lsmf_residual(S->fine.handle(), S->rhs.p, s), "residual");
@@ -1112,6 +1125,10 @@ static void newton_step(lsmf_solver *S, double t, int step, lsmf_step_stats &st)
     setup_preconditioner(S);
     hip_check(hipStreamSynchronize(s), "setup sync");
     st.time_setup_s += seconds_since(t0);
+    if (S->opt.preconditioner == LSMF_PRECONDITIONER_GMG)
+      st.time_setup_cache_s += S->mg->setup_cache_seconds();
+    else if (S->opt.preconditioner == LSMF_PRECONDITIONER_JACOBI)
+      st.time_setup_cache_s += seconds_since(t0);
     t0 = Clock::now();
     double lin_res = 0;
     const int lin = pcg(S, S->du.p, S->rhs.p, o.tol_lin * res, o.max_lin_iterations, &lin_res);

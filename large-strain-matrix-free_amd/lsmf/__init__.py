@@ -395,7 +395,8 @@ class StepStats(ctypes.Structure):
                 ("max_linear_iterations", ctypes.c_int), ("residual_0", ctypes.c_double),
                 ("residual_final", ctypes.c_double), ("time_total_s", ctypes.c_double),
                 ("time_setup_s", ctypes.c_double), ("time_linear_s", ctypes.c_double),
-                ("time_residual_s", ctypes.c_double), ("cond_number", ctypes.c_double)]
+                ("time_residual_s", ctypes.c_double), ("cond_number", ctypes.c_double),
+                ("time_setup_cache_s", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
