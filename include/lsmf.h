@@ -325,9 +325,14 @@ typedef struct {
 enum { LSMF_PRECONDITIONER_GMG = 0, LSMF_PRECONDITIONER_JACOBI = 1, LSMF_PRECONDITIONER_NONE = 2 };
 
 /* coarse-level solver: the reference's Chebyshev iteration used as a solver
- * (include/mf_elasticity.h:2087-2096, 2113), or an exact FP64 solve with the
- * inverse of the assembled coarse matrix (coarsest level up to 4096 DoFs) */
-enum { LSMF_COARSE_CHEBYSHEV = 0, LSMF_COARSE_DIRECT = 1 };
+ * (include/mf_elasticity.h:2087-2096, 2113, "MF Chebyshev coarse = true"), an
+ * exact FP64 solve with the inverse of the assembled coarse matrix (coarsest
+ * level up to 4096 DoFs), or the reference's iterative coarse solver
+ * ("MF Chebyshev coarse = false", :2115-2127): CG from zero preconditioned by
+ * the coarse level's Chebyshev smoother, to a residual reduction of 1e-3 or
+ * 1e-10 absolute, at most max(n, 100) steps.  Its iteration count depends
+ * on the data, so the V-cycle then runs without graph capture. */
+enum { LSMF_COARSE_CHEBYSHEV = 0, LSMF_COARSE_DIRECT = 1, LSMF_COARSE_CG = 2 };
 
 typedef struct {
   int newton_iterations;  /* as printed by "Converged in N Newton iterations" */

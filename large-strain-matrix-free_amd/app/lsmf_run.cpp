@@ -138,8 +138,6 @@ void setup(Problem &pr, int dim, int degree, int n_q) {
               << std::endl;
     caching = LSMF_CACHING_SCALAR;
   }
-  if (!P.mf_coarse_chebyshev)
-    throw std::runtime_error("MF Chebyshev coarse = false (CG coarse solver) is not supported");
 
   // geometry of the finest level and of the coarse mesh
   const bool gmg = P.preconditioner_type == "gmg";
@@ -220,6 +218,9 @@ void setup(Problem &pr, int dim, int degree, int n_q) {
   o.mg_precision = P.mg_level_precision == "double" ? LSMF_FP64 : LSMF_FP32;
   o.eig_cg_iterations = P.mf_chebyshev_n_cg_iterations;
   o.coarse_eig_iterations = P.mf_coarse_chebyshev_accurate_eigenval ? 0 : P.mf_chebyshev_n_cg_iterations;
+  // "MF Chebyshev coarse": the Chebyshev iteration as coarse solver, or CG
+  // preconditioned by the coarse level's smoother (include/mf_elasticity.h:2115-2175)
+  o.coarse_solver = P.mf_coarse_chebyshev ? LSMF_COARSE_CHEBYSHEV : LSMF_COARSE_CG;
   o.tol_lin = P.tol_lin;
   const size_t n_dofs = (size_t)pr.mesh->n_nodes * dim;
   o.max_lin_iterations = (int)std::min<size_t>(n_dofs * (size_t)P.max_iterations_lin, 2000000000);

@@ -258,6 +258,7 @@ template <typename T> struct Level {
   std::unique_ptr<NeoHookOperatorHIP<T>> own;
   NeoHookOperatorHIP<T> *op = nullptr;
   DeviceArray<T> sol, rhs, t, d, z;
+  DeviceArray<T> cg_r, cg_p, cg_q;  // coarse CG (LSMF_COARSE_CG, level 0 only)
   DeviceArray<double> u64;
   DeviceArray<T> tmpA, tmpB;  // transfer temporaries (from level-1 to this level)
   DevBand P[3], PT[3], R[3];
@@ -293,6 +294,11 @@ template <typename T> class GMG : public GMGBase {
       v.t.resize(n);
       v.d.resize(n);
       v.z.resize(n);
+      if (l == 0 && o.coarse_solver == LSMF_COARSE_CG) {
+        v.cg_r.resize(n);
+        v.cg_p.resize(n);
+        v.cg_q.resize(n);
+      }
       if (l < L - 1) v.u64.resize(n);
       if (l > 0) {
         const LevelGeom &c = *geoms[l - 1];
@@ -412,7 +418,7 @@ template <typename T> class GMG : public GMGBase {
       Level<T> &v = lv_[l];
       if (l == 0 && opt_.coarse_solver == LSMF_COARSE_DIRECT) {
         assemble_coarse_inverse(l < L - 1 ? lv_[0].u64.p : u_fine);
-      } else if (l == 0) {
+      } else if (l == 0 && opt_.coarse_solver == LSMF_COARSE_CHEBYSHEV) {
         const int its = opt_.coarse_eig_iterations > 0 ? opt_.coarse_eig_iterations : (int)v.g->n_dofs;
         estimate_eigenvalues(v, its);
         const double lmax = 1.2 * v.lmax_est;
@@ -438,7 +444,7 @@ template <typename T> class GMG : public GMGBase {
   // z = M r, the V-cycle preconditioner between FP64 vectors of the finest
   // level (copy_to_mg / copy_from_mg convert FP64 <-> level precision)
   void precondition(double *z, const double *r) override {
-    if (!opt_.use_graph) {
+    if (!opt_.use_graph || opt_.coarse_solver == LSMF_COARSE_CG) {
       precondition_eager(z, r);
       return;
     }
@@ -546,6 +552,8 @@ template <typename T> class GMG : public GMGBase {
       if (opt_.coarse_solver == LSMF_COARSE_DIRECT)
         hipLaunchKernelGGL(k_dense_matvec<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s_,
                            (const double *)ainv_.p, (const T *)v.rhs.p, v.sol.p, (int)n);
+      else if (opt_.coarse_solver == LSMF_COARSE_CG)
+        coarse_cg(v);
       else
         chebyshev(v, v.sol.p, v.rhs.p, true);
       return;
@@ -559,6 +567,38 @@ template <typename T> class GMG : public GMGBase {
     prolongate_add(l, v.sol.p, c.sol.p);
     zero_constrained(l, v.sol.p);
     chebyshev(v, v.sol.p, v.rhs.p, false);  // post-smoothing
+  }
+
+  // MGCoarseGridIterativeSolver (include/mf_elasticity.h:2115-2127): x = 0,
+  // then SolverCG preconditioned by the level's Chebyshev smoother (applied
+  // from zero) until |r| <= max(1e-10, 1e-3 |b|), at most max(n, 100) steps
+  void coarse_cg(Level<T> &v) {
+    const size_t n = v.g->n_dofs;
+    T *x = v.sol.p, *r = v.cg_r.p, *p = v.cg_p.p, *q = v.cg_q.p, *z = v.z.p;
+    const T *b = v.rhs.p;
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, s_, x, T(0), n);
+    hip_check(hipMemcpyAsync(r, b, n * sizeof(T), hipMemcpyDeviceToDevice, s_), "coarse cg");
+    const double r0 = std::sqrt(red_.dot(r, r, n));
+    const double tol = std::max(1e-10, 1e-3 * r0);
+    if (r0 <= tol) return;
+    const int max_it = std::max<int>((int)n, 100);
+    chebyshev(v, z, r, true);
+    hip_check(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, s_), "coarse cg");
+    double rz = red_.dot(r, z, n);
+    for (int it = 0; it < max_it; ++it) {
+      v.op->vmult(q, p);
+      const double pq = red_.dot(p, q, n);
+      if (!(pq > 0)) throw HipError(LSMF_ERR_NOT_CONVERGED, "coarse CG: operator not positive definite");
+      const double a = rz / pq;
+      const double rr = red_.cg_update(x, r, p, q, a, n);
+      if (std::sqrt(rr) <= tol) return;
+      chebyshev(v, z, r, true);
+      const double rz_new = red_.dot(r, z, n);
+      const double beta = rz_new / rz;
+      rz = rz_new;
+      hipLaunchKernelGGL(k_xpby<T>, dim3(nblk(n)), dim3(256), 0, s_, p, (const T *)z, beta, n);
+    }
+    throw HipError(LSMF_ERR_NOT_CONVERGED, "coarse CG: no convergence");
   }
 
   // CG on D^-1 A with the Lanczos tridiagonal of its coefficients

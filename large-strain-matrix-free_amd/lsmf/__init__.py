@@ -386,7 +386,7 @@ class SolverOptions(ctypes.Structure):
                 ("cond_number_cg_iterations", ctypes.c_int)]
 
 
-COARSE = {"chebyshev": 0, "direct": 1}
+COARSE = {"chebyshev": 0, "direct": 1, "cg": 2}
 PRECONDITIONER = {"gmg": 0, "jacobi": 1, "none": 2}
 
 

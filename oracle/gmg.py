@@ -12,7 +12,10 @@ Multigrid / PreconditionChebyshev / MGTransferMatrixFree / SolverCG:
     estimate (eig_cg_n_iterations); vmult from zero uses degree-1 operator
     applications, step uses degree,
   * coarse level: Chebyshev as a solver with the degree for a given
-    reduction, eigenvalue bounds from a CG run over all coarse DoFs,
+    reduction, eigenvalue bounds from a CG run over all coarse DoFs
+    ("MF Chebyshev coarse = true"), or deal.II's MGCoarseGridIterativeSolver
+    ("= false", :2115-2127): SolverCG from zero preconditioned by the coarse
+    level's Chebyshev smoother, ReductionControl(max(n, 100), 1e-10, 1e-3),
   * V-cycle: pre-smooth, residual, restrict (P^T), coarse correction,
     prolongate-add, post-smooth,
   * FP64 PCG stopping at |r| <= tol.
@@ -155,7 +158,7 @@ class GMG:
                 lv.Ainv = np.linalg.inv(0.5 * (A0 + A0.T))
                 lv.degree, lv.lmax_est, lv.lmin_est = 0, 0.0, 0.0
                 continue
-            if l == 0:
+            if l == 0 and self.coarse_solver == "chebyshev":
                 its = self.opts["coarse_eig_its"] or lv.mesh.n_dofs
                 lv.lmax_est, lv.lmin_est = self.eigen(lv, its)
                 lmax = 1.2 * lv.lmax_est
@@ -228,11 +231,39 @@ class GMG:
             x = x + d
         return x
 
+    def coarse_cg(self, lv, b):
+        """SolverCG with ReductionControl(max(n, 100), 1e-10, 1e-3) from x = 0,
+        preconditioned by the level's Chebyshev smoother (vmult from zero)"""
+        n = lv.mesh.n_dofs
+        x = np.zeros(n)
+        r = b.copy()
+        r0 = math.sqrt(r @ r)
+        tol = max(1e-10, 1e-3 * r0)
+        if r0 <= tol:
+            return x
+        z = self.chebyshev(lv, None, r, True)
+        p = z.copy()
+        rz = r @ z
+        for _ in range(max(n, 100)):
+            q = lv.op.vmult(p)
+            a = rz / (p @ q)
+            x += a * p
+            r -= a * q
+            if math.sqrt(r @ r) <= tol:
+                return x
+            z = self.chebyshev(lv, None, r, True)
+            rzn = r @ z
+            p = z + (rzn / rz) * p
+            rz = rzn
+        raise RuntimeError("coarse CG: no convergence")
+
     def vcycle(self, l, b):
         lv = self.levels[l]
         if l == 0:
             if self.coarse_solver == "direct":
                 return lv.Ainv @ b
+            if self.coarse_solver == "cg":
+                return self.coarse_cg(lv, b)
             return self.chebyshev(lv, None, b, True)
         x = self.chebyshev(lv, None, b, True)
         t = b - lv.op.vmult(x)

@@ -375,3 +375,11 @@ def test_batched_vmult_equals_single_products(dim, p, n, layout, caching, precis
         op.vmult(y, X[b].contiguous())
         assert torch.equal(Y[b], y), b
     assert float(Y[5].abs().max()) == 0.0
+    if layout == "auto" and caching == "tensor4":
+        # argument checks: no vectors, overlapping source and destination
+        import ctypes
+        assert lsmf._lib.lsmf_vmult_batched(op._h, ctypes.c_void_p(Y.data_ptr()), ctypes.c_void_p(X.data_ptr()),
+                                            0, None) == -1
+        assert lsmf._lib.lsmf_vmult_batched(op._h, ctypes.c_void_p(X[1].data_ptr()),
+                                            ctypes.c_void_p(X.data_ptr()), 2, None) == -1
+        assert b"overlap" in lsmf._lib.lsmf_last_error()

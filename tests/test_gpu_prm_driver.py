@@ -44,6 +44,22 @@ def test_blessed_output_reproduced(name, tmp_path, device):
     assert "| Linear solver" in timings
 
 
+def test_cg_coarse_solver_from_the_prm_file(tmp_path, device):
+    """cook_mf_gmg_form_1 with "MF Chebyshev coarse = false" (the
+    reference's CG coarse solver preconditioned by the coarse smoother):
+    only the preconditioner changes, so the blessed output is reproduced"""
+    src = open(os.path.join(PRM, "cook_mf_gmg_form_1.prm")).read()
+    assert "MF Chebyshev coarse" not in src
+    prm = src.replace("  set Solver type               = MF_CG\n",
+                      "  set Solver type               = MF_CG\n  set MF Chebyshev coarse = false\n")
+    assert prm != src
+    path = tmp_path / "cook_cg_coarse.prm"
+    path.write_text(prm)
+    out, timings = run_driver(str(path), str(tmp_path))
+    assert out == open(os.path.join(PRM, "cook_mf_gmg_form_1.output")).read()
+    assert "| Setup MF: GMG setup" in timings
+
+
 def test_unit_square_driver_matches_solver_api(tmp_path, device):
     """2D unit square, 2x2 coarse mesh refined 3 times, Q2, GMG, log-normal
     materials; x = 0 clamped, u_x = 0.01 per step at x = 1, two steps;

@@ -199,6 +199,43 @@ def test_direct_coarse_solve_and_graph_replays(device):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_cg_coarse_solver_matches_oracle(prec, device):
+    """LSMF_COARSE_CG (the reference's MGCoarseGridIterativeSolver: CG
+    preconditioned by the coarse Chebyshev smoother to a 1e-3 reduction):
+    the V-cycle equals the oracle's, and PCG needs the same iterations"""
+    import torch
+    from oracle.gmg import GMG
+    dim, nc, L = 3, [2, 2, 2], 3
+    log(f"[coarse cg] {prec} levels")
+    meshes = [lsmf.Mesh(dim, 2, [k << l for k in nc]) for l in range(L)]
+    cons = [m.constrained_mask({0: [0, 1, 2], 1: [0]}) for m in meshes]
+    fm = meshes[-1]
+    mu, lam, _ = lsmf.inclusion_material(fm.cell_centres(), dim, 1.0, 0.3, 4, 0.2, 10.0, 3)
+    u, _ = lsmf.smooth_displacement(fm.coords, 0.002, 1)
+    ref = GMG(meshes, cons, mu, lam, coarse_solver="cg")
+    ref.setup(u)
+    r = np.random.default_rng(1).random(fm.n_dofs)
+    r[cons[-1] == 1] = 0
+    z_ref = ref.precondition(r)
+    _, its_ref, _ = ref.pcg(ref.levels[-1].op, r, 1e-8 * np.linalg.norm(r))
+    o = lsmf.solver_options(dim=dim, degree=2, n_q_1d=3, n_cells_coarse=nc, n_levels=L, caching="tensor4",
+                            mg_caching="tensor4", mg_precision=prec, coarse_solver="cg",
+                            dirichlet={0: ([0, 1, 2], [0.0] * 3), 1: ([0], [0.0])})
+    S = lsmf.Solver(o, mu, lam)
+    S.set_state(u)
+    rd = torch.tensor(r, device=device)
+    z = torch.zeros_like(rd)
+    S.precondition(z, rd)
+    err = float(np.abs(z.cpu().numpy() - z_ref).max() / np.abs(z_ref).max())
+    log(f"  V-cycle rel diff vs oracle {err:.2e}")
+    assert err < (1e-8 if prec == "fp64" else 1e-4)
+    x = torch.zeros_like(rd)
+    its, _ = S.cg(x, rd, 1e-8 * float(np.linalg.norm(r)))
+    log(f"  PCG its {its} oracle {its_ref}")
+    assert abs(its - its_ref) <= 1
+
+
 @pytest.mark.parametrize("dim,p,nc", [(3, 2, [2, 2, 2]), (3, 4, [2, 2, 2]), (2, 2, [3, 3]), (2, 1, [5, 4])])
 def test_direct_coarse_inverse(dim, p, nc, device):
     """one-level hierarchy: the preconditioner is the exact coarse solve, so
