@@ -119,11 +119,19 @@ void setup(Problem &pr, int dim, int degree, int n_q) {
     throw std::runtime_error("Geometry type Holes (curved manifold meshes) is not supported by this build");
   if (P.type_lin == "CG")
     std::cerr << "Note: Solver type CG (assembled matrix) runs on the matrix-free operator here" << std::endl;
+  else if (P.type_lin == "MF_AD_CG")  // include/mf_ad_nh_operator.h: the same tangent by automatic differentiation
+    std::cerr << "Note: Solver type MF_AD_CG runs on the matrix-free operator here (the same tangent)" << std::endl;
   else if (P.type_lin != "MF_CG")
     throw std::runtime_error("Solver type " + P.type_lin + " is not supported: use MF_CG");
+  // gmg-gc (global coarsening, include/mf_elasticity.h:1597-1622): on the
+  // uniformly refined meshes here its levels are those of gmg
+  if (P.preconditioner_type == "gmg-gc") {
+    std::cerr << "Note: Preconditioner type gmg-gc runs as gmg (uniform refinement: the same levels)" << std::endl;
+    P.preconditioner_type = "gmg";
+  }
   if (P.preconditioner_type != "gmg" && P.preconditioner_type != "jacobi" && P.preconditioner_type != "none")
     throw std::runtime_error("Preconditioner type " + P.preconditioner_type +
-                             " is not supported: use gmg, jacobi or none");
+                             " is not supported: use gmg, gmg-gc, jacobi or none");
   int caching;
   if (P.mf_caching == "scalar") caching = LSMF_CACHING_SCALAR;
   else if (P.mf_caching == "tensor2") caching = LSMF_CACHING_TENSOR2;
@@ -526,7 +534,7 @@ void describe_setup(const Problem &pr) {
      << o.n_cells_coarse[0] << ", " << o.n_cells_coarse[1] << ", " << o.n_cells_coarse[2] << "], \"n_cells_fine\": ["
      << pr.n_fine[0] << ", " << pr.n_fine[1] << ", " << pr.n_fine[2] << "], \"caching\": " << o.caching
      << ", \"mg_precision\": " << o.mg_precision << ", \"formulation\": " << o.formulation
-     << ", \"preconditioner\": " << o.preconditioner
+     << ", \"preconditioner\": " << o.preconditioner << ", \"coarse_solver\": " << o.coarse_solver
      << ", \"relaxation\": " << o.preconditioner_relaxation << ", \"tol_lin\": " << o.tol_lin
      << ", \"max_lin_iterations\": " << o.max_lin_iterations << ", \"max_newton\": " << o.max_newton
      << ", \"tol_f\": " << o.tol_f << ", \"tol_u\": " << o.tol_u << ", \"dt\": " << o.dt << ", \"t_end\": " << o.t_end

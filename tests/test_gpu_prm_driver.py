@@ -60,6 +60,21 @@ def test_cg_coarse_solver_from_the_prm_file(tmp_path, device):
     assert "| Setup MF: GMG setup" in timings
 
 
+def test_ad_operator_and_global_coarsening_names(tmp_path, device):
+    """cook_mf_gmg_form_1 with "Solver type = MF_AD_CG" (the reference's
+    automatic-differentiation operator: the same tangent) and
+    "Preconditioner type = gmg-gc" (global coarsening: on uniformly refined
+    meshes the same levels) reproduces the blessed output"""
+    src = open(os.path.join(PRM, "cook_mf_gmg_form_1.prm")).read()
+    prm = src.replace("set Solver type               = MF_CG", "set Solver type               = MF_AD_CG")
+    prm = prm.replace("set Preconditioner type        = gmg", "set Preconditioner type        = gmg-gc")
+    assert prm.count("MF_AD_CG") == 1 and prm.count("gmg-gc") == 1
+    path = tmp_path / "cook_ad_gc.prm"
+    path.write_text(prm)
+    out, _ = run_driver(str(path), str(tmp_path))
+    assert out == open(os.path.join(PRM, "cook_mf_gmg_form_1.output")).read()
+
+
 def test_unit_square_driver_matches_solver_api(tmp_path, device):
     """2D unit square, 2x2 coarse mesh refined 3 times, Q2, GMG, log-normal
     materials; x = 0 clamped, u_x = 0.01 per step at x = 1, two steps;

@@ -214,6 +214,24 @@ def test_cook_3d_setup():
     assert js["max_lin_iterations"] == 10 * 5625
 
 
+def test_reference_option_names_mapped(tmp_path):
+    """options whose algorithm is the built one here: MF_AD_CG (the same
+    tangent by automatic differentiation) and gmg-gc (global coarsening:
+    the same levels on uniformly refined meshes) map to MF_CG / gmg; "MF
+    Chebyshev coarse = false" selects the CG coarse solver"""
+    base = os.path.join(PRM, "cook_mf_gmg_form_1.prm")
+    _, ref = describe(base)
+    assert ref["preconditioner"] == 0 and ref["coarse_solver"] == 0
+    src = open(base).read()
+    for old, new, key, want in [("= MF_CG", "= MF_AD_CG", "preconditioner", 0),
+                                ("= gmg", "= gmg-gc", "preconditioner", 0),
+                                ("= MF_CG\n", "= MF_CG\n  set MF Chebyshev coarse = false\n", "coarse_solver", 2)]:
+        assert src.count(old) == 1
+        path = write(tmp_path, "m.prm", src.replace(old, new))
+        _, js = describe(path)
+        assert js[key] == want, (new, js[key])
+
+
 def test_unsupported_settings(tmp_path):
     for extra, msg in [("subsection Geometry\n  set Type = Holes\nend\n", "Holes"),
                        ("subsection Linear solver\n  set Solver type = Direct\nend\n", "Solver type Direct"),
