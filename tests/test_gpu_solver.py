@@ -345,6 +345,36 @@ def test_config0_newton_matches_oracle(device):
     assert abs(u[mesh.boundary_nodes(1)[0] * 2] - 0.05) < 1e-12
 
 
+def test_config2_problem_fp32_levels_against_fp64_oracle(device):
+    """the 128^3 configuration's problem (nu = 0.45, 200 inclusions of
+    radius 0.03 and 10x stiffness, tensor4, x=0 clamped, 3 load steps of
+    u_x = 0.03, CG to 1e-8) on 8^3 cells: the FP32-V-cycle Newton against
+    the all-FP64 oracle's Newton (Jacobi CG): Newton counts within 1 and the
+    displacement to 1e-7 of its maximum"""
+    log("[config2 8^3] FP32-GMG Newton vs FP64 oracle")
+    cfg = lsmf.CONFIGS[2]
+    mesh, mu, lam = lsmf.build_config_mesh(2, n=8)
+    o = lsmf.solver_options(dim=3, degree=2, n_q_1d=3, n_cells_coarse=[2, 2, 2], n_levels=3, caching="tensor4",
+                            mg_caching="tensor4", mg_precision="fp32", smoother_degree=cfg["cheb_degree"],
+                            tol_lin=1e-8, tol_f=1e-8, dt=1.0 / 3, t_end=1.0 + 1e-9, coarse_solver="direct",
+                            dirichlet={0: ([0, 1, 2], [0.0] * 3), 1: ([0], [cfg["increment"]])})
+    S = lsmf.Solver(o, mu, lam)
+    stats = S.run()
+    u = S.solution()
+    assert len(stats) == 3
+    cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
+    inc = np.zeros(mesh.n_dofs)
+    inc[mesh.boundary_nodes(1) * 3] = cfg["increment"]
+    ref = oracle.OracleOperator(3, 2, mesh.cell_nodes, mesh.coords, cons, caching="tensor4")
+    ref.set_material(mu, lam)
+    its, _, u_ref = ref.newton(1.0 / 3, 1.0 + 1e-9, dirichlet_inc=inc, tol_f=1e-8, tol_lin=1e-8)
+    err = float(np.abs(u - u_ref).max() / np.abs(u_ref).max())
+    log(f"  GPU Newton {[s['newton_iterations'] for s in stats]} oracle {list(its)}, |u - u_oracle| / |u| = {err:.2e}")
+    assert len(its) == 3
+    assert all(abs(a["newton_iterations"] - b) <= 1 for a, b in zip(stats, its))
+    assert err < 1e-7
+
+
 def test_cook_jacobi_newton_matches_blessed_output_per_step(device):
     """tests/cook_mf_form_1.prm: 32x32 Q1 Cook membrane, MF_CG with
     PreconditionJacobi(0.65); driven one load step at a time through
