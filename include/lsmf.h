@@ -124,7 +124,7 @@ typedef struct {
 
 /* Cell loop layout.  AUTO: on a structured mesh (the lexicographic numbering
  * lsmf_mesh_create produces, detected from cell_nodes) cells are processed in
- * tiles (3x3x1 Q2 cells in 3D, ...) whose shared nodes are gathered once and
+ * tiles (rows of 9x1x1 Q2 cells in 3D, ...) whose shared nodes are gathered once and
  * accumulated in LDS, with the tiles coloured by parity; otherwise, and with
  * CELLS, one cell per lane group with the cells coloured (given colouring or
  * greedy).  Both give the same operator to round-off. */
