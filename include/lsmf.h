@@ -44,6 +44,7 @@ typedef int lsmf_status;
 #define LSMF_ERR_HIP (-6)
 #define LSMF_ERR_NOT_CONVERGED (-7) /* "No convergence in nonlinear solver!", include/mf_elasticity.h:2557 */
 #define LSMF_ERR_UNSUPPORTED (-8)   /* (dim, degree, n_q_1d) not instantiated, src/main.cc:102-110 */
+#define LSMF_ERR_NONPOSITIVE_DIAGONAL (-9) /* Assert(diag > 0), include/mf_nh_operator.h:1543-1547 */
 
 typedef void *lsmf_stream; /* hipStream_t */
 
@@ -64,8 +65,12 @@ enum { LSMF_FP64 = 0, LSMF_FP32 = 1 };
 
 /* vmult phase mask, the reference's MFMask (include/mf_nh_operator.h:73-86).
  * ZERO and MPI have no effect on one GPU (dst is overwritten unless add != 0;
- * there are no ghosts).  PLAIN is new: apply the operator without
- * constraints (constrained DoFs are read and written like free ones), which
+ * there are no ghosts).  As in the reference, RW gates the gather, the
+ * scatter and the identity on constrained rows (:703-705): a mask without RW
+ * does no cell work (dst = 0 in set mode, unchanged with add); RW without SF
+ * times gather and scatter only, RW | SF adds the sum factorization.  PLAIN
+ * is new: apply the whole operator without constraints (constrained DoFs are
+ * read and written like free ones; the other bits are then ignored), which
  * the residual needs for the Dirichlet lifting -K g
  * (include/mf_elasticity.h:2894-2898). */
 enum {
@@ -179,7 +184,10 @@ lsmf_status lsmf_set_linearization(lsmf_op *op, const double *u_dev, lsmf_stream
 lsmf_status lsmf_cache(lsmf_op *op, lsmf_stream stream);
 
 /* NeoHookOperator::compute_diagonal (include/mf_nh_operator.h:1503-1557):
- * exact diagonal, constrained entries 1, and its inverse. */
+ * exact diagonal, constrained entries 1, and its inverse.  DoFs that no cell
+ * references have diagonal 0 and inverse 0.  Any other entry <= 0 returns
+ * LSMF_ERR_NONPOSITIVE_DIAGONAL (the reference's Assert(diag > 0),
+ * :1543-1547); this call synchronises the stream to read that count. */
 lsmf_status lsmf_compute_diagonal(lsmf_op *op, lsmf_stream stream);
 
 /* NeoHookOperator::vmult / vmult_add / Tvmult / Tvmult_add

@@ -1,6 +1,7 @@
 // prm.cpp -- parameter-file reader and expression evaluator (host only).
 #include "prm.hpp"
 
+#include <algorithm>
 #include <cctype>
 #include <cerrno>
 #include <cmath>
@@ -322,6 +323,7 @@ struct Expression::Node {
   int var = 0;
   double (*f1)(double) = nullptr;
   double (*f2)(double, double) = nullptr;
+  int depth = 1;  // height of the subtree (bounded: evaluation and destruction recurse)
   std::shared_ptr<Node> a, b, c;
 };
 
@@ -378,12 +380,25 @@ class Parser {
   }
 
  private:
+  // nesting limits: parentheses, unary signs and ternaries recurse in the
+  // parser, and the tree's height bounds the recursion of eval / destruction
+  static constexpr int kMaxNest = 512;
+  static constexpr int kMaxDepth = 1024;
   const std::string &s_;
   const std::vector<std::string> &vars_;
   size_t i_ = 0;
+  int nest_ = 0;
+  struct Nest {
+    Parser &p;
+    explicit Nest(Parser &q) : p(q) {
+      if (++p.nest_ > kMaxNest) p.fail("nested more than " + std::to_string(kMaxNest) + " levels deep");
+    }
+    ~Nest() { --p.nest_; }
+  };
 
   [[noreturn]] void fail(const std::string &m) const {
-    throw PrmError("cannot parse expression '" + s_ + "': " + m);
+    const std::string shown = s_.size() > 120 ? s_.substr(0, 120) + "..." : s_;
+    throw PrmError("cannot parse expression '" + shown + "': " + m);
   }
   void skip() {
     while (i_ < s_.size() && std::isspace((unsigned char)s_[i_])) ++i_;
@@ -397,8 +412,11 @@ class Parser {
     }
     return false;
   }
-  static NodeP mk(Expression::Node::Op op, NodeP a = nullptr, NodeP b = nullptr, NodeP c = nullptr) {
+  NodeP mk(Expression::Node::Op op, NodeP a = nullptr, NodeP b = nullptr, NodeP c = nullptr) {
     auto n = std::make_shared<Expression::Node>();
+    for (const NodeP *ch : {&a, &b, &c})
+      if (*ch) n->depth = std::max(n->depth, (*ch)->depth + 1);
+    if (n->depth > kMaxDepth) fail("more than " + std::to_string(kMaxDepth) + " operations deep");
     n->op = op;
     n->a = std::move(a);
     n->b = std::move(b);
@@ -406,6 +424,7 @@ class Parser {
     return n;
   }
   NodeP ternary() {
+    Nest guard(*this);
     NodeP c = logic_or();
     if (eat("?")) {
       NodeP a = ternary();
@@ -455,6 +474,7 @@ class Parser {
   }
   // unary signs bind looser than ^ (-2^2 = -4)
   NodeP unary() {
+    Nest guard(*this);
     if (eat("-")) return mk(Expression::Node::NEG, unary());
     if (eat("+")) return unary();
     if (eat("!")) return mk(Expression::Node::NOT, unary());

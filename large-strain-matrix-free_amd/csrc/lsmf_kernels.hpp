@@ -1897,9 +1897,19 @@ template <typename T> __global__ void k_set_one(const int32_t *__restrict__ list
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_cons) diag[list[i]] = T(1);
 }
-template <typename T> __global__ void k_invert(const T *__restrict__ diag, T *__restrict__ inv, size_t n) {
+// inverse diagonal; entries that are not positive (the zero rows of DoFs no
+// cell references, or a tangent that is not positive definite) get inverse 0
+// and are counted in *bad (the reference asserts diag > 0 before inverting,
+// include/mf_nh_operator.h:1543-1547)
+template <typename T>
+__global__ void k_invert(const T *__restrict__ diag, T *__restrict__ inv, size_t n, int *__restrict__ bad) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) inv[i] = T(1) / diag[i];
+  if (i < n) {
+    const T d = diag[i];
+    const bool ok = d > T(0);
+    inv[i] = ok ? T(1) / d : T(0);
+    if (!ok) atomicAdd(bad, 1);
+  }
 }
 template <typename T>
 __global__ void k_scale_mul(T *__restrict__ dst, const T *__restrict__ a, const T *__restrict__ b, T omega, size_t n) {

@@ -251,8 +251,9 @@ template <int DIM, int N> struct Launch {
     if (op->n_cons > 0)
       hipLaunchKernelGGL(k_set_one<T>, dim3((op->n_cons + 255) / 256), dim3(256), 0, s, op->d_cons, op->n_cons, diag);
     const size_t n = op->n_dofs;
+    hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(64), 0, s, op->d_bad, (size_t)1);
     hipLaunchKernelGGL(k_invert<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, diag,
-                       static_cast<T *>(op->d_inv_diag), n);
+                       static_cast<T *>(op->d_inv_diag), n, op->d_bad);
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
 
@@ -276,7 +277,8 @@ template <int DIM, int N> struct Launch {
       }
     if (op->n_cons > 0)
       hipLaunchKernelGGL(k_set_one<T>, dim3((op->n_cons + 255) / 256), dim3(256), 0, s, op->d_cons, op->n_cons, diag);
-    hipLaunchKernelGGL(k_invert<T>, g, b, 0, s, diag, static_cast<T *>(op->d_inv_diag), n);
+    hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(64), 0, s, op->d_bad, (size_t)1);
+    hipLaunchKernelGGL(k_invert<T>, g, b, 0, s, diag, static_cast<T *>(op->d_inv_diag), n, op->d_bad);
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
 

@@ -723,8 +723,20 @@ lsmf_status lsmf_cache(lsmf_op *op, lsmf_stream stream) {
 lsmf_status lsmf_compute_diagonal(lsmf_op *op, lsmf_stream stream) {
   if (!op) return fail(LSMF_ERR_INVALID_ARGUMENT, "compute_diagonal: null operator");
   if (!op->cached) return fail(LSMF_ERR_NOT_CACHED, "compute_diagonal: call cache() first");
-  int rc = launch_table(op->dim, op->N)->diagonal(op, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  int rc = launch_table(op->dim, op->N)->diagonal(op, s);
   if (rc) return fail(rc, "compute_diagonal: kernel launch failed");
+  // entries <= 0 other than the zero rows of unreferenced DoFs
+  int bad = 0;
+  LSMF_HIP_TRY(hipMemcpyAsync(&bad, op->d_bad, sizeof(int), hipMemcpyDeviceToHost, s),
+               "compute_diagonal: read the non-positive count");
+  LSMF_HIP_TRY(hipStreamSynchronize(s), "compute_diagonal: synchronize");
+  bad -= op->n_orphan;
+  if (bad > 0) {
+    op->diag_ok = false;
+    return fail(LSMF_ERR_NONPOSITIVE_DIAGONAL, "No diagonal entry in a positive definite operator should be zero or "
+                                               "negative: " + std::to_string(bad) + " entries");
+  }
   op->diag_ok = true;
   return LSMF_OK;
 }
@@ -734,6 +746,17 @@ lsmf_status lsmf_vmult(lsmf_op *op, void *dst, const void *src, int add, unsigne
   if (dst == src) return fail(LSMF_ERR_INVALID_ARGUMENT, "vmult: dst and src alias");
   if (!op->cached) return fail(LSMF_ERR_NOT_CACHED, "vmult: call cache() first");
   const bool plain = (mask & LSMF_MASK_PLAIN) != 0;
+  if (!plain && !(mask & LSMF_MASK_RW)) {
+    // no read/write phase: the reference's cell loop neither reads src nor
+    // writes dst, and the constrained rows get nothing (:703-705 is under RW)
+    if (!add) {
+      const size_t bytes = op->n_dofs * elem_size(op);
+      hipLaunchKernelGGL(k_zero<unsigned char>, dim3((unsigned)((bytes + 255) / 256)), dim3(256), 0,
+                         (hipStream_t)stream, static_cast<unsigned char *>(dst), bytes);
+      LSMF_HIP_TRY(hipGetLastError(), "vmult: zero");
+    }
+    return LSMF_OK;
+  }
   unsigned ph = mask & (LSMF_MASK_SF | LSMF_MASK_QD);
   if (!(ph & LSMF_MASK_SF)) ph = 0;  // no sum factorization: gather/scatter only
   if (plain || add) ph = LSMF_MASK_SF | LSMF_MASK_QD;

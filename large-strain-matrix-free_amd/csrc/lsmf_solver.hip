@@ -878,8 +878,11 @@ double condition_number(lsmf_solver *S, const double *b, int its, double tol) {
   return lmin > 0 ? lmax / lmin : 0.0;
 }
 
-// FP64 PCG (SolverCG::solve with the chosen preconditioner)
-int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, double *res_out) {
+// FP64 PCG (SolverCG::solve with the chosen preconditioner).  Returns the
+// iteration count on convergence; throws LSMF_ERR_NOT_CONVERGED when the
+// iteration limit is reached (also with max_it = 0 and |b| > tol) or p.Ap
+// is not positive (SolverCG's breakdown check).
+int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, double *res_out, int *its_out = nullptr) {
   const size_t n = S->n;
   hipStream_t s = S->s;
   Reducer &red = *S->red;
@@ -888,6 +891,7 @@ int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, doub
   double rn = std::sqrt(red.dot(S->r.p, S->r.p, n));
   if (rn <= tol) {
     *res_out = rn;
+    if (its_out) *its_out = 0;
     return 0;
   }
   apply_preconditioner(S, S->z.p, S->r.p);
@@ -898,6 +902,12 @@ int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, doub
     ++it;
     S->fine.vmult(S->Ap.p, S->p.p);
     const double pAp = red.dot(S->p.p, S->Ap.p, n);
+    if (!(pAp > 0)) {
+      *res_out = rn;
+      if (its_out) *its_out = it;
+      throw HipError(LSMF_ERR_NOT_CONVERGED, "CG: operator not positive definite (p.Ap = " + std::to_string(pAp) +
+                                                 " at iteration " + std::to_string(it) + ")");
+    }
     const double alpha = rz / pAp;
     const double rr = red.cg_update(x, S->r.p, (const double *)S->p.p, (const double *)S->Ap.p, alpha, n);
     rn = std::sqrt(rr);
@@ -909,7 +919,11 @@ int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, doub
     hipLaunchKernelGGL(k_xpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->p.p, (const double *)S->z.p, beta, n);
   }
   *res_out = rn;
-  return rn <= tol ? it : -it;
+  if (its_out) *its_out = it;
+  if (!(rn <= tol))
+    throw HipError(LSMF_ERR_NOT_CONVERGED, "CG: no convergence in " + std::to_string(it) + " iterations (|r| = " +
+                                               std::to_string(rn) + ", tolerance " + std::to_string(tol) + ")");
+  return it;
 }
 
 void set_state(lsmf_solver *S) {
@@ -1078,9 +1092,7 @@ lsmf_status lsmf_solver_cg(lsmf_solver *S, double *x, const double *b, double to
                            double *res) {
   if (!S || !x || !b || !its || !res) return fail(LSMF_ERR_INVALID_ARGUMENT, "cg: null argument");
   return catch_all([&] {
-    const int k = pcg(S, x, b, tol, max_it, res);
-    *its = k < 0 ? -k : k;
-    if (k < 0) throw HipError(LSMF_ERR_NOT_CONVERGED, "cg: no convergence");
+    pcg(S, x, b, tol, max_it, res, its);
   });
 }
 
@@ -1173,7 +1185,6 @@ static void newton_step(lsmf_solver *S, double t, int step, lsmf_step_stats &st)
     double lin_res = 0;
     const int lin = pcg(S, S->du.p, S->rhs.p, o.tol_lin * res, o.max_lin_iterations, &lin_res);
     st.time_linear_s += seconds_since(t0);
-    if (lin < 0) throw HipError(LSMF_ERR_NOT_CONVERGED, "linear solver did not converge");
     st.linear_iterations += lin;
     st.max_linear_iterations = std::max(st.max_linear_iterations, lin);
     if (ncons)

@@ -46,7 +46,7 @@ LSMF_OK = 0
 ERRORS = {
     -1: "invalid argument", -2: "unknown caching", -3: "out of memory",
     -4: "det F <= 0", -5: "not cached", -6: "HIP error", -7: "not converged",
-    -8: "unsupported",
+    -8: "unsupported", -9: "non-positive diagonal",
 }
 CACHING = {"scalar": 0, "tensor2": 1, "tensor4": 2, "tensor4_ns": 3, "scalar_referential": 4}
 PRECISION = {"fp64": 0, "fp32": 1}

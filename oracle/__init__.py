@@ -25,6 +25,11 @@ def _cpu_has_avx512():
 
 
 def library_path():
+    # LSMF_ORACLE_LIB selects another build of the same source (the
+    # AddressSanitizer / UBSan build that the sanitizer test loads)
+    override = os.environ.get("LSMF_ORACLE_LIB")
+    if override:
+        return override
     v4 = os.path.join(_HERE, "lsmf_oracle_v4.so")
     v3 = os.path.join(_HERE, "lsmf_oracle_v3.so")
     if _cpu_has_avx512() and os.path.exists(v4):

@@ -176,13 +176,17 @@ def test_nonpositive_det_F_is_reported(device):
     assert e.value.code == -5
 
 
-def test_config2_full_size_properties(device):
-    """128^3 Q2 (50,923,779 DoF): symmetry, translations in the null space of
-    the unconstrained operator, tensor2 = tensor4 = scalar, add = set + set"""
+@pytest.mark.parametrize("cfg_id", [2, 4], ids=["config2-128^3-Q2", "config4-64^3-Q4"])
+def test_full_size_properties(cfg_id, device):
+    """the two 50,923,779-DoF configurations (128^3 Q2 and 64^3 Q4) at full
+    size: symmetry, translations in the null space of the unconstrained
+    operator, tensor2 = tensor4 = scalar, add = set + set, and the FP32
+    operator within 2e-5 of the FP64 one, for the configuration's caching"""
     import torch
-    log("[config2] 128^3 Q2 properties")
-    mesh, mu, lam = lsmf.build_config_mesh(2)
-    u, _ = lsmf.smooth_displacement(mesh.coords, 0.02, lsmf.workload.SEED + 2)
+    cfg = lsmf.CONFIGS[cfg_id]
+    log(f"[config{cfg_id}] {cfg['n']}^3 Q{cfg['degree']} properties")
+    mesh, mu, lam = lsmf.build_config_mesh(cfg_id)
+    u, _ = lsmf.smooth_displacement(mesh.coords, 0.02, lsmf.workload.SEED + cfg_id)
     cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
     assert mesh.n_dofs == 50923779
     ud = torch.tensor(u, device=device)
@@ -191,16 +195,17 @@ def test_config2_full_size_properties(device):
     g.manual_seed(4)
     v = torch.rand(mesh.n_dofs, dtype=torch.float64, device=device, generator=g)
     w = torch.rand(mesh.n_dofs, dtype=torch.float64, device=device, generator=g)
+    base_caching = cfg["caching"]
     res = {}
-    for caching in ("tensor4", "tensor2", "scalar"):
-        log(f"[config2] {caching}")
+    for caching in [base_caching] + [c for c in ("tensor4", "tensor2", "scalar") if c != base_caching]:
+        log(f"[config{cfg_id}] {caching}")
         op = lsmf.NeoHookOperator(mesh, caching=caching, constrained=cons)
         op.set_material(mu, lam)
         op.set_linearization(ud)
         op.cache()
         Av = torch.empty_like(v)
         op.vmult(Av, v)
-        if caching == "tensor4":
+        if caching == base_caching:
             Aw = torch.empty_like(v)
             op.vmult(Aw, w)
             s1, s2 = float(w @ Av), float(v @ Aw)
@@ -216,9 +221,53 @@ def test_config2_full_size_properties(device):
             assert float((acc - 2 * Av).abs().max()) <= 1e-14 * float(Av.abs().max())
         res[caching] = Av
         del op
-    base = res["tensor4"]
-    for k in ("tensor2", "scalar"):
+    base = res[base_caching]
+    for k in res:
         assert float((res[k] - base).abs().max()) < 1e-12 * float(base.abs().max()), k
+    log(f"[config{cfg_id}] {base_caching} fp32")
+    op = lsmf.NeoHookOperator(mesh, precision="fp32", caching=base_caching, constrained=cons)
+    op.set_material(mu, lam)
+    op.set_linearization(ud)
+    op.cache()
+    v32 = v.float()
+    Av32 = torch.empty_like(v32)
+    op.vmult(Av32, v32)
+    del op
+    ref = torch.empty_like(v)
+    op = lsmf.NeoHookOperator(mesh, caching=base_caching, constrained=cons)
+    op.set_material(mu, lam)
+    op.set_linearization(ud)
+    op.cache()
+    op.vmult(ref, v32.double())
+    assert float((Av32.double() - ref).abs().max()) < 2e-5 * float(ref.abs().max())
+
+
+@pytest.mark.parametrize("caching", ["tensor2", "tensor4", "scalar"])
+def test_config4_problem_small_mesh_matches_oracle(caching, device):
+    """the 64^3 Q4 configuration's inputs (nu = 0.45, log-normal mu/lambda
+    with sigma = 0.3, smooth linearisation state) on 4^3 Q4 cells: FP64
+    vmult, diagonal and residual against the oracle to 1e-12, FP32 to 2e-5"""
+    import torch
+    log(f"[config4 4^3] {caching}")
+    mesh, mu, lam = lsmf.build_config_mesh(4, n=4)
+    u, _ = lsmf.smooth_displacement(mesh.coords, 0.02, lsmf.workload.SEED + 4)
+    cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
+    ref = orc_op(mesh, mu, lam, u, cons, caching)
+    v = np.random.default_rng(3).random(mesh.n_dofs)
+    v[cons == 1] = 0.0
+    expect = ref.vmult(v)
+    for prec, tol in [("fp64", 1e-12), ("fp32", 2e-5)]:
+        op = gpu_op(mesh, mu, lam, u, cons, caching, precision=prec)
+        dt = torch.float64 if prec == "fp64" else torch.float32
+        dst = torch.full((mesh.n_dofs,), np.nan, dtype=dt, device=device)
+        op.vmult(dst, torch.tensor(v, dtype=dt, device=device))
+        vv = v if prec == "fp64" else v.astype(np.float32).astype(np.float64)
+        assert rel(dst.cpu().double().numpy(), expect if prec == "fp64" else ref.vmult(vv)) < tol, prec
+        op.compute_diagonal()
+        assert rel(op.diagonal().cpu().double().numpy(), ref.diagonal()) < tol, prec
+    r = torch.full((mesh.n_dofs,), np.nan, dtype=torch.float64, device=device)
+    gpu_op(mesh, mu, lam, u, cons, caching).residual(r)
+    assert rel(r.cpu().numpy(), ref.residual()) < 1e-12
 
 
 F0_CASES = [(3, 2, 4), (2, 1, 6), (2, 2, 5), (3, 1, 4), (3, 4, 2)]

@@ -111,31 +111,39 @@ def test_vcycle_is_spd_and_cg_converges(prec, device):
     assert its < 30
 
 
-@pytest.mark.parametrize("dim,nc,L", [(2, [2, 2], 3), (3, [2, 2, 2], 2)])
-def test_vcycle_and_pcg_match_oracle_gmg(dim, nc, L, device):
+GMG_CASES = [(2, 2, [2, 2], 3, "tensor4", "chebyshev"), (3, 2, [2, 2, 2], 2, "tensor4", "chebyshev"),
+             (3, 4, [2, 2, 2], 2, "tensor2", "chebyshev"), (3, 4, [1, 1, 1], 3, "tensor2", "direct")]
+
+
+@pytest.mark.parametrize("dim,p,nc,L,caching,coarse", GMG_CASES,
+                         ids=[f"{d}d-Q{p}-L{L}-{c}-{cs}" for d, p, _, L, c, cs in GMG_CASES])
+def test_vcycle_and_pcg_match_oracle_gmg(dim, p, nc, L, caching, coarse, device):
     """GPU V-cycle (FP64 levels) = the oracle's restated V-cycle; PCG
-    iteration counts within +-1 (FP64 levels) and +-2 (FP32 levels)"""
+    iteration counts within +-1 (FP64 levels) and +-2 (FP32 levels).  The
+    Q4 tensor2 cases are the 64^3 Q4 configuration's multigrid (its
+    prolongation / restriction bands, level tangents and eigenvalue
+    estimates) on small hierarchies"""
     import torch
     from oracle.gmg import GMG
-    log(f"[gmg parity] dim={dim} levels={L}")
-    meshes = [lsmf.Mesh(dim, 2, [k << l for k in nc]) for l in range(L)]
+    log(f"[gmg parity] dim={dim} Q{p} levels={L} {caching} coarse={coarse}")
+    meshes = [lsmf.Mesh(dim, p, [k << l for k in nc]) for l in range(L)]
     cons = [m.constrained_mask({0: list(range(dim)), 1: [0]}) for m in meshes]
     fm = meshes[-1]
     mu, lam, _ = lsmf.inclusion_material(fm.cell_centres(), dim, 1.0, 0.3, 4, 0.2, 10.0, 3)
     u, _ = lsmf.smooth_displacement(fm.coords, 0.002, 1)
-    ref = GMG(meshes, cons, mu, lam)
+    ref = GMG(meshes, cons, mu, lam, caching=caching, coarse_solver=coarse)
     ref.setup(u)
     r = np.random.default_rng(1).random(fm.n_dofs)
     r[cons[-1] == 1] = 0
     z_ref = ref.precondition(r)
     _, its_ref, _ = ref.pcg(ref.levels[-1].op, r, 1e-8 * np.linalg.norm(r))
     for prec, z_tol, it_tol in [("fp64", 1e-8, 1), ("fp32", 1e-3, 2)]:
-        o = lsmf.solver_options(dim=dim, degree=2, n_q_1d=3, n_cells_coarse=nc, n_levels=L, caching="tensor4",
-                                mg_caching="tensor4", mg_precision=prec,
+        o = lsmf.solver_options(dim=dim, degree=p, n_q_1d=p + 1, n_cells_coarse=nc, n_levels=L, caching=caching,
+                                mg_caching=caching, mg_precision=prec, coarse_solver=coarse,
                                 dirichlet={0: (list(range(dim)), [0.0] * dim), 1: ([0], [0.0])})
         S = lsmf.Solver(o, mu, lam)
         S.set_state(u)
-        for l in range(L):
+        for l in range(1 if coarse == "direct" else 0, L):
             lmax, lmin, deg = S.level_info(l)
             assert abs(lmax - ref.levels[l].lmax_est) < 1e-6 * lmax, (prec, l)
             if prec == "fp64":
@@ -370,6 +378,38 @@ def test_config2_problem_fp32_levels_against_fp64_oracle(device):
     its, _, u_ref = ref.newton(1.0 / 3, 1.0 + 1e-9, dirichlet_inc=inc, tol_f=1e-8, tol_lin=1e-8)
     err = float(np.abs(u - u_ref).max() / np.abs(u_ref).max())
     log(f"  GPU Newton {[s['newton_iterations'] for s in stats]} oracle {list(its)}, |u - u_oracle| / |u| = {err:.2e}")
+    assert len(its) == 3
+    assert all(abs(a["newton_iterations"] - b) <= 1 for a, b in zip(stats, its))
+    assert err < 1e-7
+
+
+def test_config4_problem_fp32_levels_against_fp64_oracle(device):
+    """the 64^3 Q4 configuration's problem (nu = 0.45, log-normal mu/lambda
+    with sigma = 0.3, tensor2, x=0 clamped, 3 load steps of u_x = 0.03, CG
+    to 1e-8) on 4^3 Q4 cells with two levels (coarse 2^3 Q4, exact coarse
+    solve): the FP32-V-cycle Newton against the all-FP64 oracle's Newton
+    (Jacobi CG): Newton counts within 1 and the displacement to 1e-7 of its
+    maximum"""
+    log("[config4 4^3] FP32-GMG Newton vs FP64 oracle")
+    cfg = lsmf.CONFIGS[4]
+    mesh, mu, lam = lsmf.build_config_mesh(4, n=4)
+    o = lsmf.solver_options(dim=3, degree=4, n_q_1d=5, n_cells_coarse=[2, 2, 2], n_levels=2, caching="tensor2",
+                            mg_caching="tensor2", mg_precision="fp32", tol_lin=1e-8, tol_f=1e-8, dt=1.0 / 3,
+                            t_end=1.0 + 1e-9, coarse_solver="direct",
+                            dirichlet={0: ([0, 1, 2], [0.0] * 3), 1: ([0], [cfg["increment"]])})
+    S = lsmf.Solver(o, mu, lam)
+    stats = S.run()
+    u = S.solution()
+    assert len(stats) == 3
+    cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
+    inc = np.zeros(mesh.n_dofs)
+    inc[mesh.boundary_nodes(1) * 3] = cfg["increment"]
+    ref = oracle.OracleOperator(3, 4, mesh.cell_nodes, mesh.coords, cons, caching="tensor2")
+    ref.set_material(mu, lam)
+    its, _, u_ref = ref.newton(1.0 / 3, 1.0 + 1e-9, dirichlet_inc=inc, tol_f=1e-8, tol_lin=1e-8)
+    err = float(np.abs(u - u_ref).max() / np.abs(u_ref).max())
+    log(f"  GPU Newton {[s['newton_iterations'] for s in stats]} CG {[s['linear_iterations'] for s in stats]} "
+        f"oracle {list(its)}, |u - u_oracle| / |u| = {err:.2e}")
     assert len(its) == 3
     assert all(abs(a["newton_iterations"] - b) <= 1 for a, b in zip(stats, its))
     assert err < 1e-7
