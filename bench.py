@@ -14,11 +14,23 @@ timed region.
 Prints one JSON line (stdout) with the throughput in GDoF/s, the HBM roofline
 of the fused cell kernel (algorithmic bytes of the reference's memory model,
 doc/large_strain.tex:739-753, divided by the kernel's event-timed duration)
-and the CPU oracle timed on a bounded sample of the same workload.
+and a CPU baseline timed on the host.  Beside the headline it reports
+  * the other caching variants and precisions on the same 128^3 mesh, each
+    against its roofline (scalar also against the FP64 VALU peak),
+  * the cells layout (the path for meshes without the structured numbering:
+    the same mesh with its nodes renumbered at random, and in lexicographic
+    order),
+  * the FP32-V-cycle against the all-FP64 Newton solve (3 load steps, CG to
+    1e-8) on the 128^3 Q2 and the 64^3 Q4 meshes,
+  * the 64^3 Q4 tensor2 configuration's vmult and roofline,
+  * the 48^3 Q2 differentiation workload: residual and its VJP, and the
+    gradient of the x = 1 face misfit through a Newton solve.
 """
 import argparse
+import hashlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -27,7 +39,9 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "large-strain-matrix-free_amd"))
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 METRIC = "Q2-3D tangent vmult GDoF/s; Newton-solve s, FP32-GMG vs FP64, same tol"
+KERNEL_SOURCE = os.path.join(ROOT, "large-strain-matrix-free_amd", "csrc", "lsmf_kernels.hpp")
 
 
 def progress(msg):
@@ -51,37 +65,76 @@ def algorithmic_bytes(dim, caching, precision, n_qp, n_nodes):
     return es * (per_qp * n_qp + per_node * n_nodes)
 
 
+def kernel_source_sha():
+    return hashlib.sha256(open(KERNEL_SOURCE, "rb").read()).hexdigest()[:16]
+
+
+def measured_traffic(cfg_id, caching, precision):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this kernel, if it was measured on the current
+    kernel source (else None: the figure would be stale)"""
+    f = os.path.join(ROOT, "profiles", f"traffic_config{cfg_id}_{caching}_{precision}.json")
+    if not os.path.exists(f):
+        return None, "not measured"
+    d = json.load(open(f))
+    if d.get("kernel_source_sha") != kernel_source_sha():
+        return None, f"stale: {os.path.basename(f)} was measured on another kernel source"
+    return d.get("hbm_bytes_per_launch"), os.path.relpath(f, ROOT)
+
+
+def physical_cores():
+    """physical cores of the host (lscpu: unique core,socket pairs)"""
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=30).stdout
+        pairs = {l for l in out.splitlines() if l and not l.startswith("#")}
+        if pairs:
+            return len(pairs)
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return os.cpu_count() or 1
+
+
 def cpu_baseline(cfg_id, caching, n_sample, seconds):
-    """FP64 oracle (OpenMP, sum factorization) on a smaller cube of the same
-    configuration; GDoF/s of its vmult."""
+    """the reference's MatrixFree algorithm on the host: cell batches as SIMD
+    lanes (VectorizedArray), sum factorization, the same caches, OpenMP over
+    the colour-ordered batches (oracle/cpu_baseline.cpp, a CPU restatement
+    checked against the oracle), on a smaller cube of the same
+    configuration; GDoF/s of its vmult, mean of the timed products"""
     import numpy as np
     import lsmf
-    import oracle
+    from oracle import cpu_baseline as cb
 
     mesh, mu, lam = lsmf.build_config_mesh(cfg_id, n=n_sample)
     u, _ = lsmf.smooth_displacement(mesh.coords, 0.02, lsmf.workload.SEED + cfg_id)
     cons = mesh.constrained_mask({0: list(range(mesh.dim)), 1: [0]})
-    op = oracle.OracleOperator(mesh.dim, mesh.degree, mesh.cell_nodes, mesh.coords, cons, caching=caching)
+    cores = physical_cores()
+    op = cb.BatchOperator(mesh, cons, caching=caching, threads=cores)
     op.set_material(mu, lam)
     op.set_linearization(u)
     op.cache()
     x = np.random.default_rng(0).random(mesh.n_dofs)
     x[cons == 1] = 0.0
-    op.vmult(x)
+    y = np.empty_like(x)
+    op.vmult(y, x)
     n, t0 = 0, time.perf_counter()
     while True:
-        op.vmult(x)
+        op.vmult(y, x)
         n += 1
         el = time.perf_counter() - t0
-        if el >= seconds or n >= 1000:
+        if (el >= seconds and n >= 10) or n >= 1000:
             break
     return {
         "value": mesh.n_dofs * n / el / 1e9,
         "unit": "GDoF/s",
-        "cores": oracle.num_threads(),
+        "cores": cores,
         "kind": "port",
         "sample": (f"{n_sample}^3-cell Q{mesh.degree} {caching} FP64 vmult of the same configuration "
-                   f"({mesh.n_dofs} DoF), {n} vmults in {el:.1f} s, oracle C++ (OpenMP, sum factorization)"),
+                   f"({mesh.n_dofs} DoF; GDoF/s is size-independent once the working set leaves the caches), "
+                   f"mean of {n} vmults in {el:.1f} s after one warm-up; reference-algorithm CPU restatement: "
+                   f"{op.lanes}-wide cell batches (VectorizedArray), sum factorization, {caching} cache, "
+                   f"OpenMP on {cores} physical cores (lscpu)"),
+        "published_context": "reference on 20 Cascade Lake cores: tensor4 0.182, tensor2 0.327 GDoF/s (Q2, "
+                             "Calculations/CSL_Munich/holes_3d_p2q3r3_MF_CG_gmg_*/timings.txt:51)",
     }
 
 
@@ -112,6 +165,38 @@ def newton_solve(cfg_id, n_override, mu, lam, precision, tol, coarse="direct", v
     return {"create_s": t_create, "stats": stats, "u": u, "n_levels": n_levels}
 
 
+def solve_pair(cfg_id, n_override, mu, lam, tol, coarse):
+    """FP32-V-cycle against all-FP64 Newton solve of one configuration"""
+    import numpy as np
+    import lsmf
+    cfg = lsmf.CONFIGS[cfg_id]
+    res = {}
+    for prec in ("fp32", "fp64"):
+        progress(f"config {cfg_id} Newton solve with {prec} GMG levels (tol {tol:g})")
+        res[prec] = newton_solve(cfg_id, n_override, mu, lam, prec, tol, coarse)
+        st = res[prec]["stats"]
+        progress(f"  {prec}: {sum(s['time_total_s'] for s in st):.2f} s, Newton {[s['newton_iterations'] for s in st]}"
+                 f", CG {[s['linear_iterations'] for s in st]}")
+    t32 = sum(s["time_total_s"] for s in res["fp32"]["stats"])
+    t64 = sum(s["time_total_s"] for s in res["fp64"]["stats"])
+    u32, u64 = res["fp32"]["u"], res["fp64"]["u"]
+    return {
+        "fp32_gmg_s": round(t32, 3), "fp64_gmg_s": round(t64, 3), "speedup_fp32_vs_fp64": round(t64 / t32, 3),
+        "tol": tol, "load_steps": len(res["fp32"]["stats"]), "mg_levels": res["fp32"]["n_levels"],
+        "coarse_solver": coarse,
+        "smoother": f"Chebyshev degree {cfg.get('cheb_degree', 5)}, range 20, 30 eigen-CG iterations",
+        "newton_iterations": {p: [s["newton_iterations"] for s in res[p]["stats"]] for p in res},
+        "cg_iterations": {p: [s["linear_iterations"] for s in res[p]["stats"]] for p in res},
+        "final_rel_residual": {p: [s["residual_final"] / s["residual_0"] for s in res[p]["stats"]] for p in res},
+        "setup_s": {p: round(sum(s["time_setup_s"] for s in res[p]["stats"]), 3) for p in res},
+        "setup_caches_diagonals_s": {p: round(sum(s["time_setup_cache_s"] for s in res[p]["stats"]), 3)
+                                     for p in res},
+        "linear_s": {p: round(sum(s["time_linear_s"] for s in res[p]["stats"]), 3) for p in res},
+        "rel_diff_u_fp32_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+        "timed": "Newton loop of all load steps (residual, caches, level setup, CG); hierarchy creation excluded",
+    }
+
+
 def time_ms(fn, reps, torch):
     """average ms of fn() over reps launches on the current stream (HIP events)"""
     stream = torch.cuda.current_stream()
@@ -126,9 +211,89 @@ def time_ms(fn, reps, torch):
     return e0.elapsed_time(e1) / reps
 
 
+def timed_vmults(op, x, y, steps, warmup, torch):
+    """steps products y = A x between events, each product also bracketed
+    (mean and fastest), and the fused cell kernels' event-timed launches"""
+    for _ in range(warmup):
+        op.vmult(y, x)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    op.profile_enable(True)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for k in range(steps):
+        op.vmult(y, x)
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    step_ms = [ev[k].elapsed_time(ev[k + 1]) for k in range(steps)]
+    kern_ms, n_launch = op.profile_read()
+    op.profile_enable(False)
+    return ev[0].elapsed_time(ev[steps]) / steps, min(step_ms), kern_ms, n_launch
+
+
+def vmult_record(mesh, mu, lam, ud, cons, caching, precision, steps, warmup, layout="auto", cfg_id=None,
+                 flops_per_dof=None):
+    """one operator's vmult throughput and roofline on `mesh`"""
+    import torch
+    import lsmf
+    dev = ud.device
+    op = lsmf.NeoHookOperator(mesh, precision=precision, caching=caching, constrained=cons, layout=layout,
+                              use_mesh_colouring=layout == "auto")
+    op.set_material(mu, lam)
+    op.set_linearization(ud)
+    op.cache()
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    g = torch.Generator(device=dev)
+    g.manual_seed(2354)
+    x = torch.rand(mesh.n_dofs, dtype=dt, device=dev, generator=g)
+    x[torch.tensor(cons, device=dev).bool()] = 0
+    y = torch.empty_like(x)
+    ms, ms_min, kern_ms, n_launch = timed_vmults(op, x, y, steps, warmup, torch)
+    nq = mesh.degree + 1
+    n_qp = mesh.n_cells * nq ** mesh.dim
+    alg = algorithmic_bytes(mesh.dim, caching, precision, n_qp, mesh.n_nodes)
+    launches = n_launch / steps
+    avg_s = kern_ms / n_launch / 1e3
+    achieved = alg / launches / avg_s / 1e9
+    rec = {"caching": caching, "precision": precision, "layout": layout, "ms": round(ms, 4),
+           "ms_min": round(ms_min, 4), "gdofs": round(mesh.n_dofs / ms / 1e6, 3),
+           "kernel_avg_ms": round(avg_s * 1e3, 5), "launches_per_vmult": launches,
+           "algorithmic_bytes_per_vmult": int(alg),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4)},
+           "cache_bytes": op.memory_consumption()}
+    if cfg_id is not None:
+        tr, src = measured_traffic(cfg_id, caching, precision)
+        rec["roofline"]["traffic"] = tr
+        rec["roofline"]["traffic_source"] = src
+    if flops_per_dof is not None:
+        tf = flops_per_dof * mesh.n_dofs / (kern_ms / steps / 1e3) / 1e12
+        rec["compute_roofline"] = {"bound": "fp64 valu", "achieved": round(tf, 2), "peak": FP64_PEAK_TFLOPS,
+                                   "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFLOPS, 4),
+                                   "flops_per_dof": flops_per_dof}
+    del op, x, y
+    torch.cuda.empty_cache()
+    return rec
+
+
+def permuted_mesh(mesh, seed):
+    """the same mesh with its nodes numbered in a seeded random order (no
+    structured numbering: the operator uses the cells layout)"""
+    import numpy as np
+    from types import SimpleNamespace
+    perm = np.random.default_rng(seed).permutation(mesh.n_nodes).astype(np.int32)
+    coords = np.empty_like(mesh.coords)
+    coords[perm] = mesh.coords
+    return SimpleNamespace(dim=mesh.dim, degree=mesh.degree, n_cells=mesh.n_cells, n_nodes=mesh.n_nodes,
+                           n_dofs=mesh.n_dofs, cell_nodes=perm[mesh.cell_nodes], coords=coords,
+                           cell_colour=mesh.cell_colour), perm
+
+
 def variant_sweep(cfg_id, reps):
-    """vmult throughput of every caching variant (and formulation 0) on one
-    configuration, FP64 and FP32"""
+    """vmult throughput of every caching variant (and formulation 0) on the
+    32^3 configuration, FP64 and FP32 (0.8 M DoF: launch-bound and largely
+    in the Infinity Cache, so no roofline is claimed)"""
     import torch
     import lsmf
     mesh, mu, lam = lsmf.build_config_mesh(cfg_id)
@@ -137,7 +302,7 @@ def variant_sweep(cfg_id, reps):
     dev = torch.device("cuda:0")
     ud = torch.tensor(u, device=dev)
     out = {"workload": f"config {cfg_id}: {mesh.n_cells_dir[0]}^{mesh.dim} Q{mesh.degree}, {mesh.n_dofs} DoF",
-           "gdofs": {}, "cache_bytes": {}}
+           "note": "launch-bound and cache-resident: no roofline", "gdofs": {}}
     cases = [(c, p, 1) for c in ("tensor4", "tensor2", "scalar", "tensor4_ns", "scalar_referential")
              for p in ("fp64", "fp32")] + [("scalar", "fp64", 0), ("tensor4", "fp64", 0)]
     for caching, prec, form in cases:
@@ -151,7 +316,6 @@ def variant_sweep(cfg_id, reps):
         ms = time_ms(lambda: op.vmult(y, x), reps, torch)
         key = f"{caching}_{prec}" + ("_formulation0" if form == 0 else "")
         out["gdofs"][key] = round(mesh.n_dofs / ms / 1e6, 3)
-        out["cache_bytes"][key] = op.memory_consumption()
         del op, x, y
     torch.cuda.empty_cache()
     return out
@@ -197,6 +361,48 @@ def adjoint_trade(cfg_id, reps, caching="tensor4"):
     return out
 
 
+def misfit_gradient_timing(cfg_id=3):
+    """the differentiation workload at full size: the x = 1 face
+    displacement misfit J against the solution for a second seeded
+    log-normal field; dJ/dmu, dJ/dlambda through the Newton solve (one
+    adjoint GMG-CG solve on the converged tangent and one residual VJP)"""
+    import numpy as np
+    import torch
+    import lsmf
+    from lsmf.autograd import misfit_gradient
+    cfg = lsmf.CONFIGS[cfg_id]
+    n = cfg["n"]
+    mesh = lsmf.Mesh(3, cfg["degree"], [n] * 3)
+    cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
+    n_levels = int(round(np.log2(n // 3))) + 1
+    o = lsmf.solver_options(dim=3, degree=cfg["degree"], n_q_1d=cfg["degree"] + 1, n_cells_coarse=[3] * 3,
+                            n_levels=n_levels, caching="tensor4", mg_caching="tensor4", mg_precision="fp32",
+                            coarse_solver="direct", tol_lin=1e-10, tol_f=1e-10, tol_u=1e-10, dt=1.0, t_end=1.0,
+                            dirichlet={0: ([0, 1, 2], [0.0] * 3), 1: ([0], [0.01])})
+    mu_t, lam_t = lsmf.lognormal_material(mesh.n_cells, 1.0, cfg["nu"], cfg["sigma"], lsmf.workload.SEED + 31)
+    St = lsmf.Solver(o, mu_t, lam_t)
+    St.run()
+    target = St.solution()
+    del St
+    mu, lam = lsmf.lognormal_material(mesh.n_cells, 1.0, cfg["nu"], cfg["sigma"], lsmf.workload.SEED + cfg_id)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S = lsmf.Solver(o, mu, lam)
+    stats = S.run()
+    u = S.solution()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    J, gmu, gla = misfit_gradient(S, mesh, u, mu, lam, target, mesh.boundary_nodes(1), cons, cg_tol=1e-10)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return {"workload": f"{n}^3 Q{cfg['degree']} ({mesh.n_dofs} DoF), log-normal mu/lambda sigma {cfg['sigma']}, "
+                        f"u_x = 0.01 at x = 1 in one load step, FP32-level GMG, CG to 1e-10",
+            "forward_solve_s": round(t1 - t0, 3), "gradient_s": round(t2 - t1, 3),
+            "newton_iterations": [s["newton_iterations"] for s in stats],
+            "J": J, "grad_norm_mu": float(np.linalg.norm(gmu)), "grad_norm_lambda": float(np.linalg.norm(gla)),
+            "timed": "forward: solver creation + Newton; gradient: adjoint CG on the converged tangent + VJP"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,14 +413,14 @@ def main():
     ap.add_argument("--caching", default=None)
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-n", type=int, default=40)
+    ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="skip the FP32-GMG vs FP64-GMG Newton solves")
     ap.add_argument("--solve-tol", type=float, default=1e-8)
     ap.add_argument("--coarse", default="direct", choices=["direct", "chebyshev"],
                     help="coarse-level solver of the V-cycle (chebyshev = the reference's choice)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the caching-variant sweep (32^3 Q2) and the adjoint time/memory trade (48^3 Q2)")
+                    help="skip everything beside the headline vmult (and the solve unless --no-solve)")
     ap.add_argument("--solve-only", default=None, choices=["fp32", "fp64"],
                     help="run only the Newton solve with this level precision and print its statistics "
                          "as one JSON line (no vmult timing; for kernel traces of the solve)")
@@ -265,30 +471,8 @@ def main():
     x = torch.rand(mesh.n_dofs, dtype=dt, device=dev, generator=g)
     x[torch.tensor(cons, device=dev).bool()] = 0
     y = torch.empty_like(x)
-    progress("warmup")
-    for _ in range(args.warmup):
-        op.vmult(y, x)
-    torch.cuda.synchronize()
-
     progress(f"timing {args.steps} vmults")
-    stream = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    op.profile_enable(True)
-    step_ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    torch.cuda.synchronize()
-    e0.record(stream)
-    step_ev[0].record(stream)
-    for k in range(args.steps):
-        op.vmult(y, x)
-        step_ev[k + 1].record(stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    total_ms = e0.elapsed_time(e1)
-    step_ms = [step_ev[k].elapsed_time(step_ev[k + 1]) for k in range(args.steps)]
-    kern_ms, n_launch = op.profile_read()
-    op.profile_enable(False)
-    ms_per_step = total_ms / args.steps
+    ms_per_step, ms_min, kern_ms, n_launch = timed_vmults(op, x, y, args.steps, args.warmup, torch)
     if args.calibrate_copy:
         progress(f"copy calibration: 3 x {x.numel() * x.element_size()} bytes read + written")
         for _ in range(3):
@@ -302,59 +486,79 @@ def main():
     bytes_per_launch = alg_bytes / launches_per_vmult
     avg_launch_s = kern_ms / n_launch / 1e3
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_config{args.config}_{caching}_{args.precision}.json")
-    if args.n is None and os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+    traffic, traffic_src = (None, "not measured")
+    if args.n is None:
+        traffic, traffic_src = measured_traffic(args.config, caching, args.precision)
+    del op, x, y
+    torch.cuda.empty_cache()
 
-    del op, ud, x, y
+    extras = not args.no_extras and args.n is None
+    variants_128 = cells = None
+    if extras:
+        variants_128 = {"workload": f"config {args.config}: {mesh.n_cells_dir[0]}^{mesh.dim} Q{mesh.degree}, "
+                                    f"{mesh.n_dofs} DoF, tiled layout", "records": []}
+        for c2, p2 in (("tensor2", "fp64"), ("tensor4", "fp32"), ("tensor2", "fp32"), ("scalar", "fp64")):
+            if (c2, p2) == (caching, args.precision):
+                continue
+            progress(f"variant {c2} {p2} on the config {args.config} mesh")
+            flops = SCALAR_FLOPS_PER_DOF.get((mesh.dim, mesh.degree)) if c2 == "scalar" else None
+            rec = vmult_record(mesh, mu, lam, ud, cons, c2, p2, 10, 2, cfg_id=args.config, flops_per_dof=flops)
+            progress(f"  {rec['ms']} ms, frac {rec['roofline']['frac']}")
+            variants_128["records"].append(rec)
+        progress("cells layout: lexicographic numbering and a random node renumbering")
+        cells = {"workload": f"config {args.config} mesh, {caching} {args.precision}, one cell per lane group, "
+                             f"greedy cell colouring", "records": []}
+        rec = vmult_record(mesh, mu, lam, ud, cons, caching, args.precision, 10, 2, layout="cells")
+        rec["numbering"] = "lexicographic"
+        cells["records"].append(rec)
+        pm, perm = permuted_mesh(mesh, 7)
+        pcons = np.empty_like(cons)
+        pcons.reshape(-1, mesh.dim)[perm] = cons.reshape(-1, mesh.dim)
+        upd = torch.empty_like(ud)
+        upd.view(-1, mesh.dim)[torch.tensor(perm, device=dev, dtype=torch.int64)] = ud.view(-1, mesh.dim)
+        rec = vmult_record(pm, mu, lam, upd, pcons, caching, args.precision, 5, 1, layout="cells")
+        rec["numbering"] = "random permutation of the nodes (seed 7)"
+        cells["records"].append(rec)
+        del pm, perm, pcons, upd
+        progress(f"  {[r['ms'] for r in cells['records']]} ms")
+    del ud
     torch.cuda.empty_cache()
 
     solve = None
     if not args.no_solve and "load_steps" in cfg:
-        res = {}
-        for prec in ("fp32", "fp64"):
-            progress(f"Newton solve with {prec} GMG levels (tol {args.solve_tol:g})")
-            res[prec] = newton_solve(args.config, args.n, mu, lam, prec, args.solve_tol, args.coarse)
-            st = res[prec]["stats"]
-            progress(f"  {prec}: {sum(s['time_total_s'] for s in st):.2f} s, Newton {[s['newton_iterations'] for s in st]}"
-                     f", CG {[s['linear_iterations'] for s in st]}")
-        t32 = sum(s["time_total_s"] for s in res["fp32"]["stats"])
-        t64 = sum(s["time_total_s"] for s in res["fp64"]["stats"])
-        u32, u64 = res["fp32"]["u"], res["fp64"]["u"]
-        solve = {
-            "fp32_gmg_s": round(t32, 3), "fp64_gmg_s": round(t64, 3), "speedup_fp32_vs_fp64": round(t64 / t32, 3),
-            "tol": args.solve_tol, "load_steps": len(res["fp32"]["stats"]), "mg_levels": res["fp32"]["n_levels"],
-            "coarse_solver": args.coarse,
-            "smoother": f"Chebyshev degree {cfg.get('cheb_degree', 5)}, range 20, 30 eigen-CG iterations",
-            "newton_iterations": {p: [s["newton_iterations"] for s in res[p]["stats"]] for p in res},
-            "cg_iterations": {p: [s["linear_iterations"] for s in res[p]["stats"]] for p in res},
-            "final_rel_residual": {p: [s["residual_final"] / s["residual_0"] for s in res[p]["stats"]] for p in res},
-            "setup_s": {p: round(sum(s["time_setup_s"] for s in res[p]["stats"]), 3) for p in res},
-            "setup_caches_diagonals_s": {p: round(sum(s["time_setup_cache_s"] for s in res[p]["stats"]), 3)
-                                         for p in res},
-            "linear_s": {p: round(sum(s["time_linear_s"] for s in res[p]["stats"]), 3) for p in res},
-            "rel_diff_u_fp32_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
-            "timed": "Newton loop of all load steps (residual, caches, level setup, CG); hierarchy creation excluded",
-        }
-        del res
+        solve = solve_pair(args.config, args.n, mu, lam, args.solve_tol, args.coarse)
 
-    variants = adjoint = None
-    if not args.no_extras and args.n is None:
+    config4 = variants = adjoint = gradient = None
+    if extras:
+        if args.config != 4:
+            progress("config 4: 64^3 Q4 tensor2")
+            m4, mu4, lam4 = lsmf.build_config_mesh(4)
+            u4, _ = lsmf.smooth_displacement(m4.coords, 0.02, lsmf.workload.SEED + 4)
+            cons4 = m4.constrained_mask({0: [0, 1, 2], 1: [0]})
+            ud4 = torch.tensor(u4, device=dev)
+            del u4
+            config4 = {"workload": f"config 4: 64^3 Q4 (5^3 Gauss points), {m4.n_dofs} DoF, nu 0.45, log-normal "
+                                   f"mu/lambda sigma 0.3",
+                       "vmult": vmult_record(m4, mu4, lam4, ud4, cons4, "tensor2", "fp64", 20, 3, cfg_id=4),
+                       "vmult_fp32": vmult_record(m4, mu4, lam4, ud4, cons4, "tensor2", "fp32", 10, 2, cfg_id=4)}
+            progress(f"  vmult {config4['vmult']['ms']} ms, frac {config4['vmult']['roofline']['frac']}")
+            del ud4
+            torch.cuda.empty_cache()
+            if not args.no_solve:
+                config4["newton_solve"] = solve_pair(4, None, mu4, lam4, args.solve_tol, args.coarse)
+            progress("config 4 adjoint time/memory trade (tensor2)")
+            config4["adjoint"] = adjoint_trade(4, 10, "tensor2")
+            del m4, mu4, lam4, cons4
+        progress("config 3: residual VJP trade and the misfit gradient through a Newton solve (48^3 Q2)")
+        adjoint = adjoint_trade(3, 20)
+        gradient = misfit_gradient_timing(3)
+        progress(f"  gradient {gradient}")
         progress("caching-variant sweep on config 1 (32^3 Q2)")
         variants = variant_sweep(1, 20)
-        progress(f"  GDoF/s {variants['gdofs']}")
-        if args.config == 4:
-            progress("adjoint time/memory trade on config 4 (64^3 Q4, tensor2)")
-            adjoint = adjoint_trade(4, 10, "tensor2")
-        else:
-            progress("adjoint time/memory trade on config 3 (48^3 Q2)")
-            adjoint = adjoint_trade(3, 20)
-        progress(f"  {adjoint}")
 
     cpu = None
     if not args.no_cpu:
-        progress(f"CPU baseline: oracle on {args.cpu_n}^3 sample for ~{args.cpu_seconds:.0f} s")
+        progress(f"CPU baseline: {args.cpu_n}^3 sample for ~{args.cpu_seconds:.0f} s")
         cpu = cpu_baseline(args.config, caching, args.cpu_n, args.cpu_seconds)
 
     workload = (f"config {args.config}: {mesh.n_cells_dir[0]}^{mesh.dim} Q{mesh.degree} {caching} "
@@ -367,7 +571,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 4),
-        "ms_per_step_min": round(min(step_ms), 4) if step_ms else None,
+        "ms_per_step_min": round(ms_min, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -377,16 +581,32 @@ def main():
                    "cells": mesh.n_cells, "dofs": mesh.n_dofs, "qp": n_qp, "caching": caching,
                    "precision": args.precision, "nu": cfg["nu"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_vmult (fused gather/SF/tangent/integrate/scatter, one launch per colour)",
                      "kernel_avg_ms": round(avg_launch_s * 1e3, 5), "launches_per_step": launches_per_vmult,
                      "algorithmic_bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu,
         "newton_solve": solve,
-        "caching_variants": variants,
-        "adjoint": adjoint,
+        "variants_same_mesh": variants_128,
+        "cells_layout": cells,
+        "config4": config4,
+        "config3_adjoint": adjoint,
+        "config3_misfit_gradient": gradient,
+        "caching_variants_32": variants,
     }
     print(json.dumps(line), flush=True)
+
+
+# FP64 floating-point operations per DoF of one scalar-variant vmult
+# (the kernel's FP64 VALU adds, multiplies and fused multiply-adds counted
+# by rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 (FMA = 2), divided by the DoFs):
+# profiles/k_vmult_config2_scalar_fp64.md
+SCALAR_FLOPS_PER_DOF = {}
+_sf = os.path.join(ROOT, "profiles", "flops_scalar_fp64.json")
+if os.path.exists(_sf):
+    for k, v in json.load(open(_sf)).get("flops_per_dof", {}).items():
+        d, p = k.split("_")
+        SCALAR_FLOPS_PER_DOF[(int(d), int(p))] = v
 
 
 if __name__ == "__main__":

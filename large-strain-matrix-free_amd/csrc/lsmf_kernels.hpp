@@ -34,6 +34,18 @@
 namespace lsmf {
 
 constexpr int kWG = 256;
+
+// Tiled-layout kinds (the TILE template argument of the cell kernels):
+// 0 = cells layout, 1 = tiles of one 256-thread workgroup, 2 = wide tiles of
+// one 512-thread workgroup.  3D Q4 operators with tensor2 or tensor4 caching
+// use wide tiles: four 125-point cells, 4x1x1, gather 425 nodes for the 256
+// they own (1.66 touches per node, 17-node rows of 408 bytes in FP64) where
+// 2x1x1 tiles gather 225 for 128 (1.76 touches, 9-node rows of 216 bytes):
+// 64^3 Q4 tensor2 FP64 1.73 -> 1.57 ms, tensor4 2.65 -> 2.52 ms.  Variants
+// with larger register footprints (scalar, tensor4_ns, scalar_referential:
+// 150-230 VGPRs) would run one 512-thread workgroup per CU and keep 256.
+constexpr int kTileWide = 2;
+template <int TILE> __host__ __device__ constexpr int tile_threads() { return TILE == kTileWide ? 2 * kWG : kWG; }
 constexpr uint32_t kNodeMask = (1u << 28) - 1u;
 constexpr int kConsShift = 28;  // bits 28..30: component c constrained
 constexpr uint32_t kFirstBit = 1u << 31;  // this cell writes (not adds) first
@@ -151,19 +163,20 @@ template <int DIM, int N> struct CellShape {
 // Tile of the tiled layout: TX x TY x TZ cells of a structured mesh, whose
 // (TX p + 1)(TY p + 1)(TZ p + 1) nodes are gathered once into LDS and whose
 // contributions are summed there before one write per node.  TX TY TZ = CPW.
-template <int DIM, int N> struct TileShape {
-  static constexpr int CPW = CellShape<DIM, N>::CPW;
+template <int DIM, int N, int WG_ = kWG> struct TileShape {
+  static constexpr int WG = WG_;
+  static constexpr int CPW = WG / ipow(N, DIM);
   // 3D Q2: one row of 9 cells (19 x 3 x 3 nodes) rather than 3 x 3 x 1 (7 x 7 x 3): more node
   // touches (2.5 instead of 2.0 per node) but 19-node rows (456 B in FP64) fill their cache lines
   // where 7-node rows (168 B) span two or three half-used ones
-  static constexpr int TX = DIM == 3 ? (N == 2 ? 4 : N == 3 ? 9 : 2) : (N == 2 ? 8 : N == 3 ? 7 : 5);
-  static constexpr int TY = DIM == 3 ? (N == 2 ? 4 : N == 3 ? 1 : 1) : (N == 2 ? 8 : N == 3 ? 4 : 2);
+  static constexpr int TX = DIM == 3 ? (N == 2 ? 4 : N == 3 ? 9 : CPW) : (N == 2 ? 8 : N == 3 ? 7 : 5);
+  static constexpr int TY = DIM == 3 ? (N == 2 ? 4 : N == 3 ? CPW / 9 : 1) : (N == 2 ? 8 : N == 3 ? 4 : 2);
   static constexpr int TZ = DIM == 3 ? (N == 2 ? 2 : 1) : 1;
   static constexpr int P = N - 1;
   static constexpr int NX = TX * P + 1, NY = TY * P + 1, NZ = DIM == 3 ? TZ * P + 1 : 1;
   static constexpr int TN = NX * NY * NZ;
   static_assert(TX * TY * TZ == CPW, "tile must hold the workgroup's cells");
-  static_assert(TN <= kWG, "one lane per tile node");
+  static_assert(TN <= WG, "one lane per tile node");
 };
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;  // tile node outside the mesh
 
@@ -518,7 +531,7 @@ template <int N, typename T> struct SFLine {
   // at quadrature point r of this lane's slot (lanes with on), left in
   // array d * 3 + c.  A lane reads only its own point r at the end, so the
   // caller may overwrite point r of any array (integrate) without a barrier.
-  template <int CPW>
+  template <int CPW, int WG = kWG>
   static __device__ __forceinline__ void evaluate(const T *sf, T *lds, int t, bool on, int slot, int r,
                                                   T (&G)[3][3]) {
     const T *S = sf, *D = sf + EO::PK;
@@ -526,7 +539,7 @@ template <int N, typename T> struct SFLine {
     constexpr int E1 = 3 * NQP, E2 = 6 * NQP;  // offsets of stage arrays 1, 2
     // x: e0 = S u, e1 = D u along x; line l = (i1, i2)
 #pragma unroll 1
-    for (int task = t; task < TASKS; task += kWG) {
+    for (int task = t; task < TASKS; task += WG) {
       int l;
       T *p = task_base(lds, task, l) + l * N;
       T x[N], y[N];
@@ -540,7 +553,7 @@ template <int N, typename T> struct SFLine {
     lds_barrier();
     // y: e0 <- S e0 (value), e1 <- S e1 (d/dx), e2 = D e0 (d/dy); line (i0, i2)
 #pragma unroll 1
-    for (int task = t; task < TASKS; task += kWG) {
+    for (int task = t; task < TASKS; task += WG) {
       int l;
       T *p = task_base(lds, task, l) + (l % N) + NL * (l / N);
       T x0[N], x1[N], y[N];
@@ -557,7 +570,7 @@ template <int N, typename T> struct SFLine {
     lds_barrier();
     // z: G_c0 = S e1, G_c1 = S e2, G_c2 = D e0 into arrays e0, e1, e2; line (i0, i1)
 #pragma unroll 1
-    for (int task = t; task < TASKS; task += kWG) {
+    for (int task = t; task < TASKS; task += WG) {
       int l;
       T *p = task_base(lds, task, l) + l;
       T y0[N], y1[N], y2[N], y[N];
@@ -584,7 +597,7 @@ template <int N, typename T> struct SFLine {
   // fluxes Phi[c][d] at quadrature point r of this lane's slot -> out[c] =
   // sum_q sum_d dphi_r/dxi_d Phi_cd at support point r (read back from
   // array c after the last barrier).
-  template <int CPW>
+  template <int CPW, int WG = kWG>
   static __device__ __forceinline__ void integrate(const T *sf, T *lds, int t, bool on, int slot, int r,
                                                    const T (&Phi)[3][3], T (&out)[3]) {
     const T *S = sf, *D = sf + EO::PK;
@@ -600,7 +613,7 @@ template <int N, typename T> struct SFLine {
     lds_barrier();
     // z^T: e0 <- S^T e0, e1 <- S^T e1, e2 <- D^T e2; line (i0, i1)
 #pragma unroll 1
-    for (int task = t; task < TASKS; task += kWG) {
+    for (int task = t; task < TASKS; task += WG) {
       int l;
       T *p = task_base(lds, task, l) + l;
       T f0[N], f1[N], f2[N], y[N];
@@ -617,7 +630,7 @@ template <int N, typename T> struct SFLine {
     lds_barrier();
     // y^T: e0 <- S^T e2 + D^T e1, e1 <- S^T e0; line (i0, i2)
 #pragma unroll 1
-    for (int task = t; task < TASKS; task += kWG) {
+    for (int task = t; task < TASKS; task += WG) {
       int l;
       T *p = task_base(lds, task, l) + (l % N) + NL * (l / N);
       T r10[N], r01[N], r00[N], y[N], z[N];
@@ -635,7 +648,7 @@ template <int N, typename T> struct SFLine {
     lds_barrier();
     // x^T: e0 <- S^T e0 + D^T e1; line (i1, i2)
 #pragma unroll 1
-    for (int task = t; task < TASKS; task += kWG) {
+    for (int task = t; task < TASKS; task += WG) {
       int l;
       T *p = task_base(lds, task, l) + l * N;
       T p0[N], p1[N], y[N], z[N];
@@ -967,7 +980,7 @@ template <typename Number> struct VmultParams {
 // (StaticMappingQ1, :1828-1829).  Returns GX = Grad_xi X at this lane's
 // quadrature point from the support-point coordinates; A (and B for the
 // ping-pong sum factorization) are free again on return.
-template <int DIM, int N, typename T, bool LINE>
+template <int DIM, int N, typename T, bool LINE, int CPW = CellShape<DIM, N>::CPW, int WG = kWG>
 __device__ __forceinline__ void geometry_gradient(const double *__restrict__ coords, const T *sf, const T *tab,
                                                   T *lds, T *A, T *B, bool on, bool active, uint32_t code, int t,
                                                   int slot, int r, int i0, int i1, int i2, T (&GX)[DIM][DIM]) {
@@ -978,7 +991,7 @@ __device__ __forceinline__ void geometry_gradient(const double *__restrict__ coo
     for (int c = 0; c < DIM; ++c) A[c * NQP + r] = T(coords[node * DIM + c]);
   lds_barrier();
   if constexpr (LINE && DIM == 3)
-    SFLine<N, T>::template evaluate<CellShape<DIM, N>::CPW>(sf, lds, t, on, slot, r, GX);
+    SFLine<N, T>::template evaluate<CPW, WG>(sf, lds, t, on, slot, r, GX);
   else
     SF<DIM, N, T>::template evaluate<DIM>(tab, A, B, on, r, i0, i1, i2, GX);
   lds_barrier();
@@ -1000,7 +1013,7 @@ __device__ __forceinline__ void eulerian_metric(const T (&GX)[DIM][DIM], const T
 // LINE: 3D gradient by the line-per-lane sum factorization (P.sf set,
 // lds = the workgroup's slot buffers, t = thread, slot = this lane's cell).
 // Gu = Grad_xi u (rows: components, columns: xi directions).
-template <int DIM, int N, typename T, bool LINE = false>
+template <int DIM, int N, typename T, bool LINE = false, int CPW = CellShape<DIM, N>::CPW, int WG = kWG>
 __device__ __forceinline__ void kinematics_prologue(const VmultParams<T> &P, const T *tab, T *A, T *B, T *Xc,
                                                     bool on, bool active, uint32_t code, int r, int i0, int i1, int i2,
                                                     T (&F)[DIM][DIM], T (&Jri)[DIM][DIM], T &detJr, T (&Gu)[DIM][DIM],
@@ -1017,7 +1030,7 @@ __device__ __forceinline__ void kinematics_prologue(const VmultParams<T> &P, con
   }
   lds_barrier();
   if constexpr (LINE && DIM == 3)
-    SFLine<N, T>::template evaluate<CellShape<DIM, N>::CPW>(P.sf, lds, t, on, slot, r, Gu);
+    SFLine<N, T>::template evaluate<CPW, WG>(P.sf, lds, t, on, slot, r, Gu);
   else
     SF<DIM, N, T>::template evaluate<DIM>(tab, A, B, on, r, i0, i1, i2, Gu);
   T Jr[DIM][DIM];
@@ -1081,15 +1094,18 @@ __device__ __forceinline__ void kinematics_collocation(const VmultParams<T> &P, 
 //        DoF stores (no separate zeroing pass)
 //   CONSTR: constrained DoFs read as zero and are skipped on write
 //   PH: phase bits (SF, QD) for breakdown timings
-//   TILE: tiled layout -- one workgroup per tile of a structured mesh; the
-//        tile's nodes are gathered once into LDS and the cells' results are
-//        summed there (fixed order) before one read-modify-write per node
+//   TILE: 0 = cells layout; 1 (2: wide tiles) = tiled layout -- one
+//        workgroup per tile of a structured mesh; the tile's nodes are
+//        gathered once into LDS and the cells' results are summed there
+//        (fixed order) before one read-modify-write per node
 // ------------------------------------------------------------------------
-template <int DIM, int N, typename T, int VAR, bool ADD, bool CONSTR, unsigned PH, bool TILE>
-__global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
+template <int DIM, int N, typename T, int VAR, bool ADD, bool CONSTR, unsigned PH, int TILE>
+__global__ void __launch_bounds__(tile_threads<TILE>()) k_vmult(const VmultParams<T> P) {
   using CS = CellShape<DIM, N>;
-  using TS = TileShape<DIM, N>;
-  constexpr int NQP = CS::NQP, CPW = CS::CPW, NB = CS::NB, NC = CS::NC;
+  // threads and cells of this workgroup: a tile, or CS::CPW cells
+  constexpr int WG = tile_threads<TILE>();
+  using TS = TileShape<DIM, N, WG>;
+  constexpr int NQP = CS::NQP, CPW = WG / NQP, NB = CS::NB, NC = CS::NC;
   constexpr int TN = TILE ? TS::TN : 1;
   constexpr bool RECOMP = (VAR == V_RESIDUAL || VAR == V_ADJOINT);  // no cache, c from ln J in FP64
   constexpr int NF = CacheFields<DIM, (RECOMP ? V_SCALAR : VAR)>::value;
@@ -1199,10 +1215,11 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
     // referential: the Q1 metric only, whatever the mesh (:1255-1400)
     kinematics_collocation<DIM, N, T>(P, tab, A, Xc, on, active, code, r, i0, i1, i2, uq, F, Jri, detJr, Gu);
   } else if constexpr (KIN && (PH & PH_QD)) {
-    kinematics_prologue<DIM, N, T, true>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr, Gu,
-                                         lds, t, slot);
+    kinematics_prologue<DIM, N, T, true, CPW, WG>(P, tab, A, B, Xc, on, active, code, r, i0, i1, i2, F, Jri, detJr,
+                                                  Gu, lds, t, slot);
     if (VAR == V_SCALAR && P.curved)
-      geometry_gradient<DIM, N, T, true>(P.coords, P.sf, tab, lds, A, B, on, active, code, t, slot, r, i0, i1, i2, GX);
+      geometry_gradient<DIM, N, T, true, CPW, WG>(P.coords, P.sf, tab, lds, A, B, on, active, code, t, slot, r, i0,
+                                                  i1, i2, GX);
   }
 
   T G[DIM][DIM];
@@ -1240,7 +1257,7 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
       for (int c = 0; c < DIM; ++c) out[c] = A[c * NQP + r];
     } else {
       if constexpr (DIM == 3)
-        SFLine<N, T>::template evaluate<CPW>(P.sf, lds, t, on, slot, r, G);
+        SFLine<N, T>::template evaluate<CPW, WG>(P.sf, lds, t, on, slot, r, G);
       else
         SF<DIM, N, T>::template evaluate<DIM>(tab, A, B, on, r, i0, i1, i2, G);
     }
@@ -1374,7 +1391,7 @@ __global__ void __launch_bounds__(kWG) k_vmult(const VmultParams<T> P) {
       }  // formulation 1
     }
     if constexpr (DIM == 3)
-      SFLine<N, T>::template integrate<CPW>(P.sf, lds, t, on, slot, r, Phi, out);
+      SFLine<N, T>::template integrate<CPW, WG>(P.sf, lds, t, on, slot, r, Phi, out);
     else
       SF<DIM, N, T>::integrate(tab, A, B, on, r, i0, i1, i2, Phi, out);
     if constexpr (VAR == V_ADJOINT) {
@@ -1664,10 +1681,10 @@ template <int DIM, int N, typename T, int VAR> __global__ void __launch_bounds__
 //                          + c2 gamma_c^2               (tensor2 / scalar)
 // where s~ is the Voigt vector of sym(e_c (x) gamma) with doubled shears.
 // ------------------------------------------------------------------------
-template <int DIM, int N, typename T, int VAR, bool TILE>
+template <int DIM, int N, typename T, int VAR, int TILE>
 __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
   using CS = CellShape<DIM, N>;
-  using TS = TileShape<DIM, N>;
+  using TS = TileShape<DIM, N, tile_threads<TILE>()>;  // the layout's tiles (this kernel: 256 threads)
   constexpr int NQP = CS::NQP, CPW = CS::CPW, NB = CS::NB, NC = CS::NC;
   constexpr int NV = Voigt<DIM>::NV;
   // the scalar variant evaluates the tensor4 fields per point (either
@@ -1681,8 +1698,17 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
   // the kinematic prologue's scratch (SF ping-pong, corners) is dead once the
   // per-point fields are written, so it shares the fields' region
   constexpr int QDN = CPW * NF * NQP > SFN + XN ? CPW * NF * NQP : SFN + XN;
-  __shared__ T lds[QDN + TABN];
+  // a tile of TC cells is done in ROUNDS rounds of CPW cells (3D Q4 tiles
+  // hold four cells, twice this kernel's); the per-lane results of all the
+  // tile's cells then live in their own region, with one round in the
+  // fields' region once those are dead
+  constexpr int TC = TILE ? TS::CPW : CPW;
+  constexpr int ROUNDS = (TC + CPW - 1) / CPW;
+  constexpr int RESN = ROUNDS > 1 ? TC * NQP * DIM : 0;  // TC cells' results
+  static_assert(ROUNDS > 1 || CPW * NQP * DIM <= QDN, "tile results fit the fields' region");
+  __shared__ T lds[QDN + TABN + RESN];
   T *qd_all = lds;
+  T *res = ROUNDS > 1 ? lds + QDN + TABN : lds;
   T *tab = lds + QDN;
   T *sf_all = lds;
   T *X_all = sf_all + SFN;
@@ -1693,14 +1719,16 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const bool on = slot < CPW;
   const int tile = P.cell_begin + L;
+  int ex = 0, ey = 0, ez = 0;
+  for (int round = 0; round < ROUNDS; ++round) {
+  const int ts = round * CPW + slot;  // this lane's cell within the tile
   int cell;
   bool active;
-  int ex = 0, ey = 0, ez = 0;
   if constexpr (TILE) {
     const int first = P.tinfo[2 * (size_t)tile], ext = P.tinfo[2 * (size_t)tile + 1];
     ex = ext & 0xff, ey = (ext >> 8) & 0xff, ez = ext >> 16;
-    const int sx = slot % TS::TX, sy = (slot / TS::TX) % TS::TY, sz = slot / (TS::TX * TS::TY);
-    active = on && sx < ex && sy < ey && sz < ez;
+    const int sx = ts % TS::TX, sy = (ts / TS::TX) % TS::TY, sz = ts / (TS::TX * TS::TY);
+    active = on && ts < TC && sx < ex && sy < ey && sz < ez;
     cell = first + sx + ex * (sy + ey * sz);
   } else {
     cell = P.cell_begin + L * CPW + slot;
@@ -1823,15 +1851,27 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
   }
   if constexpr (TILE) {
     lds_barrier();  // every lane is done with the per-point fields
-    T *res = lds;   // CPW * NQP * DIM values, within the fields' region
     if (on)
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) res[t * DIM + c] = active ? acc[c] : T(0);
+      for (int c = 0; c < DIM; ++c)
+        if (ts < TC) res[(ts * NQP + r) * DIM + c] = active ? acc[c] : T(0);
     lds_barrier();
-    if (t < TS::TN) {
-      const uint32_t tcode = P.tnodes[(size_t)tile * TS::TN + t];
+  } else {
+    const size_t node = code & kNodeMask;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      if ((code >> (kConsShift + c)) & 1u) continue;
+      T *d = P.dst + node * DIM + c;
+      if (code & kFirstBit) *d = acc[c];
+      else *d += acc[c];
+    }
+  }
+  }  // rounds
+  if constexpr (TILE) {
+    for (int nd = t; nd < TS::TN; nd += kWG) {
+      const uint32_t tcode = P.tnodes[(size_t)tile * TS::TN + nd];
       if (tcode != kNoNode) {
-        const int X = t % TS::NX, Y = (t / TS::NX) % TS::NY, Z = t / (TS::NX * TS::NY);
+        const int X = nd % TS::NX, Y = (nd / TS::NX) % TS::NY, Z = nd / (TS::NX * TS::NY);
         constexpr int PP = N - 1;
         T sum[DIM];
 #pragma unroll
@@ -1857,15 +1897,6 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
         }
       }
     }
-    return;
-  }
-  const size_t node = code & kNodeMask;
-#pragma unroll
-  for (int c = 0; c < DIM; ++c) {
-    if ((code >> (kConsShift + c)) & 1u) continue;
-    T *d = P.dst + node * DIM + c;
-    if (code & kFirstBit) *d = acc[c];
-    else *d += acc[c];
   }
 }
 

@@ -11,6 +11,31 @@ namespace lsmf {
 template <int DIM, int N> struct Launch {
   using CS = CellShape<DIM, N>;
 
+  // wide tiles exist for 3D Q4 and the variants that run on tensor2 /
+  // tensor4 operators (their products, residual and residual VJP)
+  template <int VAR> static constexpr bool wide_ok() {
+    return DIM == 3 && (N == 5 || N == 3) &&
+           (VAR == V_TENSOR2 || VAR == V_TENSOR4 || VAR == V_RESIDUAL || VAR == V_ADJOINT);
+  }
+
+  // one colour's fused cell kernel: n tiles (tiled layouts) or n cells, nv
+  // vectors (grid y)
+  template <typename T, int VAR, bool ADD, bool CONSTR, unsigned PH>
+  static int launch_vmult(const lsmf_op *op, const VmultParams<T> &P, int n, unsigned nv, hipStream_t s) {
+    if (op->tile_kind == kTileWide) {
+      if constexpr (wide_ok<VAR>())
+        hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, ADD, CONSTR, PH, kTileWide>), dim3(n, nv),
+                           dim3(tile_threads<kTileWide>()), 0, s, P);
+      else
+        return LSMF_ERR_UNSUPPORTED;
+    } else if (op->tile_kind == 1) {
+      hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, ADD, CONSTR, PH, 1>), dim3(n, nv), dim3(kWG), 0, s, P);
+    } else {
+      hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, ADD, CONSTR, PH, 0>), dim3(grid_for(n), nv), dim3(kWG), 0, s, P);
+    }
+    return 0;
+  }
+
   template <typename T> static VmultParams<T> params(const lsmf_op *op, int k) {
     VmultParams<T> P{};
     P.idx = op->d_idx;
@@ -70,7 +95,7 @@ template <int DIM, int N> struct Launch {
 
   // one launch per colour; the tiled layout launches one workgroup per tile
   template <typename T, int VAR, bool ADD, bool CONSTR, unsigned PH>
-  static void run_colours(const lsmf_op *op, T *dst, const T *src, hipStream_t s) {
+  static int run_colours(const lsmf_op *op, T *dst, const T *src, hipStream_t s) {
     const int nc = n_colours(op);
     for (int k = 0; k < nc; ++k) {
       VmultParams<T> P = params<T>(op, k);
@@ -79,13 +104,11 @@ template <int DIM, int N> struct Launch {
       const int n = P.cell_end - P.cell_begin;
       if (n <= 0) continue;
       if (op->prof) op->prof->before(s);
-      if (op->tiled)
-        hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, ADD, CONSTR, PH, true>), dim3(n), dim3(kWG), 0, s, P);
-      else
-        hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, ADD, CONSTR, PH, false>), dim3(grid_for(n)), dim3(kWG), 0, s,
-                           P);
+      const int rc = launch_vmult<T, VAR, ADD, CONSTR, PH>(op, P, n, 1, s);
       if (op->prof) op->prof->after(s);
+      if (rc) return rc;
     }
+    return 0;
   }
 
   template <typename T, int VAR>
@@ -96,22 +119,24 @@ template <int DIM, int N> struct Launch {
     constexpr unsigned FULL = PH_SF | PH_QD;
     zero_if_constrained(op, dst, add, s);
     if (!add) zero_orphans(op, dst, s);
+    int rc;
     if (constr) {
       if (ph == FULL) {
-        if (add) run_colours<T, VAR, true, true, FULL>(op, dst, src, s);
-        else run_colours<T, VAR, false, true, FULL>(op, dst, src, s);
+        if (add) rc = run_colours<T, VAR, true, true, FULL>(op, dst, src, s);
+        else rc = run_colours<T, VAR, false, true, FULL>(op, dst, src, s);
       } else if (ph == PH_SF) {
-        run_colours<T, VAR, false, true, PH_SF>(op, dst, src, s);
+        rc = run_colours<T, VAR, false, true, PH_SF>(op, dst, src, s);
       } else {
-        run_colours<T, VAR, false, true, 0u>(op, dst, src, s);
+        rc = run_colours<T, VAR, false, true, 0u>(op, dst, src, s);
       }
       if (op->n_cons > 0)
         hipLaunchKernelGGL(k_constrained_identity<T>, dim3((op->n_cons + 255) / 256), dim3(256), 0, s, op->d_cons,
                            op->n_cons, src, dst, add ? 1 : 0);
     } else {
-      if (add) run_colours<T, VAR, true, false, FULL>(op, dst, src, s);
-      else run_colours<T, VAR, false, false, FULL>(op, dst, src, s);
+      if (add) rc = run_colours<T, VAR, true, false, FULL>(op, dst, src, s);
+      else rc = run_colours<T, VAR, false, false, FULL>(op, dst, src, s);
     }
+    if (rc) return rc;
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
 
@@ -140,11 +165,8 @@ template <int DIM, int N> struct Launch {
         P.vstride = n;
         const int nb = P.cell_end - P.cell_begin;
         if (nb <= 0) continue;
-        if (op->tiled)
-          hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, false, true, FULL, true>), dim3(nb, nv), dim3(kWG), 0, s, P);
-        else
-          hipLaunchKernelGGL((k_vmult<DIM, N, T, VAR, false, true, FULL, false>), dim3(grid_for(nb), nv), dim3(kWG),
-                             0, s, P);
+        const int rc = launch_vmult<T, VAR, false, true, FULL>(op, P, nb, nv, s);
+        if (rc) return rc;
       }
       if (op->n_cons > 0)
         hipLaunchKernelGGL(k_constrained_identity_batch<T>, dim3((op->n_cons + 255) / 256, nv), dim3(256), 0, s,
@@ -241,10 +263,16 @@ template <int DIM, int N> struct Launch {
       if (op->caching == LSMF_CACHING_SCALAR_REFERENTIAL) P.curved = 0;
       const int n = P.cell_end - P.cell_begin;
       if (n <= 0) continue;
-      if (op->tiled)
-        hipLaunchKernelGGL((k_diagonal<DIM, N, T, VAR, true>), dim3(n), dim3(kWG), 0, s, P);
-      else
-        hipLaunchKernelGGL((k_diagonal<DIM, N, T, VAR, false>), dim3(grid_for(n)), dim3(kWG), 0, s, P);
+      if (op->tile_kind == kTileWide) {
+        if constexpr (wide_ok<VAR>())
+          hipLaunchKernelGGL((k_diagonal<DIM, N, T, VAR, kTileWide>), dim3(n), dim3(kWG), 0, s, P);
+        else
+          return LSMF_ERR_UNSUPPORTED;
+      } else if (op->tile_kind == 1) {
+        hipLaunchKernelGGL((k_diagonal<DIM, N, T, VAR, 1>), dim3(n), dim3(kWG), 0, s, P);
+      } else {
+        hipLaunchKernelGGL((k_diagonal<DIM, N, T, VAR, 0>), dim3(grid_for(n)), dim3(kWG), 0, s, P);
+      }
     }
     T *diag = static_cast<T *>(op->d_diag);
     zero_orphans(op, diag, s);
@@ -272,7 +300,8 @@ template <int DIM, int N> struct Launch {
       for (int c = 0; c < DIM; ++c) {
         hipLaunchKernelGGL(k_probe_set<T>, g, b, 0, s, x, op->d_node_colour, op->d_fixed, k, c, DIM, n);
         zero_if_constrained(op, y, false, s);
-        run_colours<T, VAR, false, true, FULL>(op, y, x, s);
+        const int rc = run_colours<T, VAR, false, true, FULL>(op, y, x, s);
+        if (rc) return rc;
         hipLaunchKernelGGL(k_probe_take<T>, g, b, 0, s, diag, y, op->d_node_colour, op->d_fixed, k, c, DIM, n);
       }
     if (op->n_cons > 0)
@@ -304,7 +333,8 @@ template <int DIM, int N> struct Launch {
     constexpr unsigned FULL = PH_SF | PH_QD;
     zero_if_constrained(op, rhs, false, s);
     zero_orphans(op, rhs, s);
-    run_colours<double, V_RESIDUAL, false, false, FULL>(op, rhs, nullptr, s);
+    const int rc = run_colours<double, V_RESIDUAL, false, false, FULL>(op, rhs, nullptr, s);
+    if (rc) return rc;
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
 
@@ -313,7 +343,8 @@ template <int DIM, int N> struct Launch {
     constexpr unsigned FULL = PH_SF | PH_QD;
     zero_if_constrained(op, du, false, s);
     zero_orphans(op, du, s);
-    run_colours<double, V_ADJOINT, false, false, FULL>(op, du, w, s);
+    const int rc = run_colours<double, V_ADJOINT, false, false, FULL>(op, du, w, s);
+    if (rc) return rc;
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
 

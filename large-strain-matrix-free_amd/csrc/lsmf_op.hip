@@ -177,17 +177,19 @@ bool support_points_curved(const lsmf_mesh_desc *m, int dim, int p, int nqp) {
 struct TileDims {
   int tx, ty, tz, nx, ny, nz, tn;
 };
-template <int D, int NN> TileDims tile_dims_of() {
-  using TS = TileShape<D, NN>;
+template <int D, int NN, int WG> TileDims tile_dims_of() {
+  using TS = TileShape<D, NN, WG>;
   return {TS::TX, TS::TY, TS::TZ, TS::NX, TS::NY, TS::NZ, TS::TN};
 }
-bool tile_dims(int dim, int N, TileDims &t) {
-  if (dim == 2 && N == 2) t = tile_dims_of<2, 2>();
-  else if (dim == 2 && N == 3) t = tile_dims_of<2, 3>();
-  else if (dim == 2 && N == 5) t = tile_dims_of<2, 5>();
-  else if (dim == 3 && N == 2) t = tile_dims_of<3, 2>();
-  else if (dim == 3 && N == 3) t = tile_dims_of<3, 3>();
-  else if (dim == 3 && N == 5) t = tile_dims_of<3, 5>();
+bool tile_dims(int dim, int N, int kind, TileDims &t) {
+  if (dim == 2 && N == 2) t = tile_dims_of<2, 2, kWG>();
+  else if (dim == 2 && N == 3) t = tile_dims_of<2, 3, kWG>();
+  else if (dim == 2 && N == 5) t = tile_dims_of<2, 5, kWG>();
+  else if (dim == 3 && N == 2) t = tile_dims_of<3, 2, kWG>();
+  else if (dim == 3 && N == 3 && kind == kTileWide) t = tile_dims_of<3, 3, 2 * kWG>();
+  else if (dim == 3 && N == 3) t = tile_dims_of<3, 3, kWG>();
+  else if (dim == 3 && N == 5 && kind == kTileWide) t = tile_dims_of<3, 5, 2 * kWG>();
+  else if (dim == 3 && N == 5) t = tile_dims_of<3, 5, kWG>();
   else return false;
   return true;
 }
@@ -200,7 +202,7 @@ int build_tiles(lsmf_op *op, const lsmf_mesh_desc *mesh, const long long n[3], s
                 std::vector<int32_t> &tinfo) {
   const int dim = op->dim, p = op->degree;
   TileDims td;
-  if (!tile_dims(dim, op->N, td)) return fail(LSMF_ERR_UNSUPPORTED, "tiles: no tile shape");
+  if (!tile_dims(dim, op->N, op->tile_kind, td)) return fail(LSMF_ERR_UNSUPPORTED, "tiles: no tile shape");
   const long long Nx = n[0] * p + 1, Ny = n[1] * p + 1, Nz = dim == 3 ? n[2] * p + 1 : 1;
   const long long TXn = (n[0] + td.tx - 1) / td.tx, TYn = (n[1] + td.ty - 1) / td.ty,
                   TZn = (n[2] + td.tz - 1) / td.tz;
@@ -432,6 +434,15 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   std::vector<int32_t> tinfo;
   long long sn[3] = {1, 1, 1};
   op->tiled = opt->layout == LSMF_LAYOUT_AUTO && nnc == 0 && structured_dims(mesh, dim, p, nqp, sn);
+  // wide (512-thread, four-cell) tiles for 3D Q4 tensor2 / tensor4 products
+#ifndef LSMF_WIDE_Q2
+#define LSMF_WIDE_Q2 0
+#endif
+  op->tile_kind = !op->tiled ? 0
+                  : (dim == 3 && (op->N == 5 || (op->N == 3 && LSMF_WIDE_Q2)) &&
+                     (op->caching == LSMF_CACHING_TENSOR2 || op->caching == LSMF_CACHING_TENSOR4))
+                      ? kTileWide
+                      : 1;
   if (op->tiled) {
     const int rc = build_tiles(op, mesh, sn, tnodes, tinfo);
     if (rc) {

@@ -81,6 +81,7 @@ struct lsmf_op {
   // k are [tile_ptr[k], tile_ptr[k+1]); each tile's cells are consecutive
   // sorted cells
   bool tiled = false;
+  int tile_kind = 0;  // TILE argument of the cell kernels: 0 cells, 1 tiles, 2 wide tiles
   std::vector<int> tile_ptr;
   uint32_t *d_tile_nodes = nullptr;  // [n_tiles][TN] node | constraint bits | first-touch bit, or kNoNode
   int32_t *d_tile_info = nullptr;    // [n_tiles][2] first sorted cell, extents ex | ey << 8 | ez << 16

@@ -158,20 +158,75 @@ static Band transpose_band(const Band &P) {
   return t;
 }
 
+// a band on the device: rows trimmed of their zero end taps (start moved
+// past leading zeros, len up to the last nonzero), weights padded to the
+// widest row W, in FP64 and FP32 (the FP32 levels transfer in FP32)
 struct DevBand {
-  DeviceArray<int32_t> start;
+  DeviceArray<int32_t> start, len;
   DeviceArray<double> w;
+  DeviceArray<float> wf;
   int W = 0, n_out = 0, n_in = 0;
   void set(const Band &b) {
-    W = b.W;
     n_out = b.n_out;
     n_in = b.n_in;
-    start.resize(b.start.size());
-    start.upload(b.start.data(), b.start.size());
-    w.resize(b.w.size());
-    w.upload(b.w.data(), b.w.size());
+    std::vector<int32_t> st(n_out), ln(n_out);
+    for (int o = 0; o < n_out; ++o) {
+      const double *r = &b.w[(size_t)o * b.W];
+      int lo = 0, hi = b.W;
+      while (lo < hi && r[lo] == 0.0) ++lo;
+      while (hi > lo && r[hi - 1] == 0.0) --hi;
+      st[o] = b.start[o] + lo;
+      ln[o] = hi - lo;
+    }
+    W = 1;
+    for (int o = 0; o < n_out; ++o) W = std::max(W, ln[o]);
+    std::vector<double> wd((size_t)n_out * W, 0.0);
+    for (int o = 0; o < n_out; ++o) {
+      const int lo = st[o] - b.start[o];
+      for (int k = 0; k < ln[o]; ++k) wd[(size_t)o * W + k] = b.w[(size_t)o * b.W + lo + k];
+      // padded taps read valid input (weight 0) when the row ends early
+      if (st[o] + W > n_in) {
+        const int shift = st[o] + W - n_in;
+        for (int k = ln[o] - 1; k >= 0; --k) wd[(size_t)o * W + k + shift] = wd[(size_t)o * W + k];
+        for (int k = 0; k < shift; ++k) wd[(size_t)o * W + k] = 0.0;
+        st[o] -= shift;
+        ln[o] += shift;
+      }
+    }
+    std::vector<float> wfh(wd.begin(), wd.end());
+    start.resize(n_out);
+    start.upload(st.data(), st.size());
+    len.resize(n_out);
+    len.upload(ln.data(), ln.size());
+    w.resize(wd.size());
+    w.upload(wd.data(), wd.size());
+    wf.resize(wfh.size());
+    wf.upload(wfh.data(), wfh.size());
+  }
+  template <typename T> const T *weights() const {
+    if constexpr (std::is_same<T, double>::value) return w.p;
+    else return wf.p;
   }
 };
+
+template <typename T, int C>
+static void band_launch(const DevBand &b, int d, const int *dims, const T *in, T *out, bool add, hipStream_t s) {
+  int od[3] = {dims[0], dims[1], dims[2]};
+  od[d] = b.n_out;
+  if (od[1] > 65535 || od[2] > 65535) throw HipError(LSMF_ERR_UNSUPPORTED, "transfer: more than 65535 nodes per direction");
+  const dim3 grid((unsigned)((od[0] * C + 255) / 256), (unsigned)od[1], (unsigned)od[2]);
+  const T *wt = b.weights<T>();
+  const int a = add ? 1 : 0;
+  if (d == 0)
+    hipLaunchKernelGGL((k_band<T, 0, C>), grid, dim3(256), 0, s, in, out, b.start.p, b.len.p, wt, b.W, dims[0], dims[1],
+                       b.n_out, a);
+  else if (d == 1)
+    hipLaunchKernelGGL((k_band<T, 1, C>), grid, dim3(256), 0, s, in, out, b.start.p, b.len.p, wt, b.W, dims[0], dims[1],
+                       b.n_out, a);
+  else
+    hipLaunchKernelGGL((k_band<T, 2, C>), grid, dim3(256), 0, s, in, out, b.start.p, b.len.p, wt, b.W, dims[0], dims[1],
+                       b.n_out, a);
+}
 
 // apply one band per direction (tensor product), in has node dims nin[0..dim)
 template <typename T>
@@ -182,11 +237,9 @@ static void tensor_apply(const DevBand *const *bands, int dim, const int *nin, c
   for (int d = 0; d < dim; ++d) {
     T *dst = (d == dim - 1) ? out : ((d % 2 == 0) ? tmpA : tmpB);
     const DevBand &b = *bands[d];
-    int od[3] = {dims[0], dims[1], dims[2]};
-    od[d] = b.n_out;
-    const size_t total = (size_t)od[0] * od[1] * od[2] * dim;
-    hipLaunchKernelGGL(k_apply_1d<T>, dim3(nblk(total)), dim3(256), 0, s, cur, dst, b.start.p, b.w.p, b.W, d, dims[0],
-                       dims[1], dims[2], b.n_out, dim, (add && d == dim - 1) ? 1 : 0);
+    const bool a = add && d == dim - 1;
+    if (dim == 3) band_launch<T, 3>(b, d, dims, cur, dst, a, s);
+    else band_launch<T, 2>(b, d, dims, cur, dst, a, s);
     dims[d] = b.n_out;
     cur = dst;
   }
@@ -881,7 +934,7 @@ double condition_number(lsmf_solver *S, const double *b, int its, double tol) {
 // FP64 PCG (SolverCG::solve with the chosen preconditioner).  Returns the
 // iteration count on convergence; throws LSMF_ERR_NOT_CONVERGED when the
 // iteration limit is reached (also with max_it = 0 and |b| > tol) or p.Ap
-// is not positive (SolverCG's breakdown check).
+// is zero or not finite.
 int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, double *res_out, int *its_out = nullptr) {
   const size_t n = S->n;
   hipStream_t s = S->s;
@@ -902,11 +955,15 @@ int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, doub
     ++it;
     S->fine.vmult(S->Ap.p, S->p.p);
     const double pAp = red.dot(S->p.p, S->Ap.p, n);
-    if (!(pAp > 0)) {
+    // breakdown: p.Ap zero or not finite (deal.II's ExcDivideByZero).  A
+    // tiny negative p.Ap from round-off late in a long solve is continued,
+    // as SolverCG does: the reference's 3D Q1 Cook output
+    // (tests/cook_3d_lin_mf.output) takes such steps
+    if (pAp == 0.0 || !std::isfinite(pAp)) {
       *res_out = rn;
       if (its_out) *its_out = it;
-      throw HipError(LSMF_ERR_NOT_CONVERGED, "CG: operator not positive definite (p.Ap = " + std::to_string(pAp) +
-                                                 " at iteration " + std::to_string(it) + ")");
+      throw HipError(LSMF_ERR_NOT_CONVERGED, "CG: breakdown (p.Ap = " + std::to_string(pAp) + " at iteration " +
+                                                 std::to_string(it) + ")");
     }
     const double alpha = rz / pAp;
     const double rr = red.cg_update(x, S->r.p, (const double *)S->p.p, (const double *)S->Ap.p, alpha, n);

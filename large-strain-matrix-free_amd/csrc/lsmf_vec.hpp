@@ -326,33 +326,38 @@ __global__ void __launch_bounds__(256) k_bgj_update(double *__restrict__ A, int 
   }
 }
 
-// One banded 1D operator applied along direction `dir` of a node array of
-// shape (n2, n1, n0, ncomp) (x fastest); out has n_dir replaced by n_out:
-//   out[.., o, ..] = sum_k w[o * W + k] * in[.., start[o] + k, ..]
-// add != 0: out += (used for prolongate-and-add).
-template <typename T>
-__global__ void k_apply_1d(const T *__restrict__ in, T *__restrict__ out, const int32_t *__restrict__ start,
-                           const double *__restrict__ w, int W, int dir, int n0, int n1, int n2, int n_out, int ncomp,
-                           int add) {
-  // output shape
-  const int m0 = dir == 0 ? n_out : n0, m1 = dir == 1 ? n_out : n1, m2 = dir == 2 ? n_out : n2;
-  const size_t total = (size_t)m0 * m1 * m2 * ncomp;
-  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= total) return;
-  const int c = gid % ncomp;
-  size_t rest = gid / ncomp;
-  const int i0 = rest % m0;
-  rest /= m0;
-  const int i1 = rest % m1;
-  const int i2 = (int)(rest / m1);
-  const int o = dir == 0 ? i0 : (dir == 1 ? i1 : i2);
+// One banded 1D operator along direction DIR of a node array of shape
+// (n2, n1, n0, C) (x fastest, C components interleaved); the output has
+// n_DIR replaced by n_out:
+//   out[.., o, ..] = sum_{k < len[o]} w[o * W + k] in[.., start[o] + k, ..]
+// (add != 0: out +=, the prolongate-and-add of the V-cycle).  Grid: x
+// covers the m0 * C outputs of one x-line, 256 per block; y and z are the
+// output's i1 and i2, so no index is divided but by the constant C.  Along
+// y and z the row o is the same for the whole block (its start, length and
+// weights are scalar loads) and the taps loop over its nonzero length;
+// along x the lanes differ in o and loop over the padded width W (zero
+// weights past a row's length).  T is the level's precision: FP32 levels
+// transfer in FP32 with FP32 weights.
+template <typename T, int DIR, int C>
+__global__ void __launch_bounds__(256) k_band(const T *__restrict__ in, T *__restrict__ out,
+                                              const int32_t *__restrict__ start, const int32_t *__restrict__ len,
+                                              const T *__restrict__ w, int W, int n0, int n1, int n_out, int add) {
+  const int m0 = DIR == 0 ? n_out : n0, m1 = DIR == 1 ? n_out : n1;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= m0 * C) return;
+  const int i0 = e / C, c = e - i0 * C;
+  const int o = DIR == 0 ? i0 : (DIR == 1 ? (int)blockIdx.y : (int)blockIdx.z);
   const int s = start[o];
-  const size_t stride = dir == 0 ? (size_t)ncomp : (dir == 1 ? (size_t)n0 * ncomp : (size_t)n0 * n1 * ncomp);
-  const int j0 = dir == 0 ? s : i0, j1 = dir == 1 ? s : i1, j2 = dir == 2 ? s : i2;
-  const T *src = in + (((size_t)j2 * n1 + j1) * n0 + j0) * ncomp + c;
-  double acc = 0.0;
-  for (int k = 0; k < W; ++k) acc += w[o * W + k] * double(src[(size_t)k * stride]);
-  out[gid] = add ? T(double(out[gid]) + acc) : T(acc);
+  const int j0 = DIR == 0 ? s : i0, j1 = DIR == 1 ? s : (int)blockIdx.y, j2 = DIR == 2 ? s : (int)blockIdx.z;
+  const size_t line = (size_t)n0 * C;
+  const T *src = in + ((size_t)j2 * n1 + j1) * line + (size_t)j0 * C + c;
+  const size_t stride = DIR == 0 ? (size_t)C : (DIR == 1 ? line : line * n1);
+  const T *wr = w + (size_t)o * W;
+  const int nk = DIR == 0 ? W : len[o];
+  T acc = T(0);
+  for (int k = 0; k < nk; ++k) acc = fma(wr[k], src[(size_t)k * stride], acc);
+  T *dst = out + ((size_t)blockIdx.z * m1 + blockIdx.y) * ((size_t)m0 * C) + e;
+  *dst = add ? *dst + acc : acc;
 }
 
 }  // namespace lsmf

@@ -315,7 +315,8 @@ def test_formulation0_rejected_where_unsupported(device):
             lsmf.NeoHookOperator(mesh, caching=caching, formulation=0)
 
 
-LAYOUT_CASES = [(3, 2, 4), (3, 2, 5), (3, 2, 7), (3, 2, 11), (2, 1, 9), (2, 2, 9), (3, 1, 5), (3, 4, 3), (2, 4, 7)]
+LAYOUT_CASES = [(3, 2, 4), (3, 2, 5), (3, 2, 7), (3, 2, 11), (2, 1, 9), (2, 2, 9), (3, 1, 5), (3, 4, 3), (3, 4, 5),
+                (3, 4, 6), (2, 4, 7)]
 
 
 @pytest.mark.parametrize("dim,p,n", LAYOUT_CASES, ids=[f"{d}d-Q{p}-n{n}" for d, p, n in LAYOUT_CASES])
