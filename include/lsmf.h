@@ -183,6 +183,20 @@ lsmf_status lsmf_set_linearization(lsmf_op *op, const double *u_dev, lsmf_stream
  * call synchronises the stream to read that count). */
 lsmf_status lsmf_cache(lsmf_op *op, lsmf_stream stream);
 
+/* lsmf_cache without any allocation: the same cache kernel and result,
+ * with the det F <= 0 counter in a file-scope device array of the library
+ * (64 ints, of which this call uses one) instead of the operator's own
+ * counter.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (HIP graph set-up code, serving loops, harnesses
+ * that forbid hipMalloc).  It calls no hipMalloc / hipFree: the array's
+ * address is looked up once per device, on the first call with that device
+ * current.  Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (several host threads or streams) race; calls in
+ * sequence are safe (each clears the counter first).  Its scratch does not
+ * grow with the operator, so no size can exceed it.  Like lsmf_cache it
+ * synchronises the stream to read the count. */
+lsmf_status lsmf_cache_noalloc(lsmf_op *op, lsmf_stream stream);
+
 /* NeoHookOperator::compute_diagonal (include/mf_nh_operator.h:1503-1557):
  * exact diagonal, constrained entries 1, and its inverse.  DoFs that no cell
  * references have diagonal 0 and inverse 0.  Any other entry <= 0 returns

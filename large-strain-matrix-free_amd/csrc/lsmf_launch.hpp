@@ -216,7 +216,7 @@ template <int DIM, int N> struct Launch {
     }
   }
 
-  template <typename T, int VAR> static int cache_t(const lsmf_op *op, hipStream_t s) {
+  template <typename T, int VAR> static int cache_t(const lsmf_op *op, int *bad, hipStream_t s) {
     CacheParams P{};
     P.idx = op->d_idx;
     P.tab = op->d_tab64;
@@ -229,24 +229,24 @@ template <int DIM, int N> struct Launch {
     P.n_cells = op->n_cells;
     P.formulation = op->formulation;
     P.curved = op->curved;
-    P.bad = op->d_bad;
+    P.bad = bad;
     hipLaunchKernelGGL((k_cache<DIM, N, T, VAR>), dim3(grid_for(op->n_cells)), dim3(kWG), 0, s, P);
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
 
-  static int cache(const lsmf_op *op, hipStream_t s) {
+  static int cache(const lsmf_op *op, int *bad, hipStream_t s) {
     const bool f64 = op->precision == LSMF_FP64;
     switch (op->caching) {
       case LSMF_CACHING_TENSOR4:
-        return f64 ? cache_t<double, V_TENSOR4>(op, s) : cache_t<float, V_TENSOR4>(op, s);
+        return f64 ? cache_t<double, V_TENSOR4>(op, bad, s) : cache_t<float, V_TENSOR4>(op, bad, s);
       case LSMF_CACHING_TENSOR2:
-        return f64 ? cache_t<double, V_TENSOR2>(op, s) : cache_t<float, V_TENSOR2>(op, s);
+        return f64 ? cache_t<double, V_TENSOR2>(op, bad, s) : cache_t<float, V_TENSOR2>(op, bad, s);
       case LSMF_CACHING_SCALAR:
-        return f64 ? cache_t<double, V_SCALAR>(op, s) : cache_t<float, V_SCALAR>(op, s);
+        return f64 ? cache_t<double, V_SCALAR>(op, bad, s) : cache_t<float, V_SCALAR>(op, bad, s);
       case LSMF_CACHING_TENSOR4_NS:
-        return f64 ? cache_t<double, V_TENSOR4_NS>(op, s) : cache_t<float, V_TENSOR4_NS>(op, s);
+        return f64 ? cache_t<double, V_TENSOR4_NS>(op, bad, s) : cache_t<float, V_TENSOR4_NS>(op, bad, s);
       case LSMF_CACHING_SCALAR_REFERENTIAL:
-        return f64 ? cache_t<double, V_SCALAR_REF>(op, s) : cache_t<float, V_SCALAR_REF>(op, s);
+        return f64 ? cache_t<double, V_SCALAR_REF>(op, bad, s) : cache_t<float, V_SCALAR_REF>(op, bad, s);
       default:
         return fail(LSMF_ERR_UNKNOWN_CACHING, "cache: caching variant not available on the GPU");
     }

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -272,6 +273,59 @@ __global__ void k_convert_f64_f32(const double *__restrict__ a, float *__restric
   if (i < n) b[i] = float(a[i]);
 }
 
+}  // namespace
+}  // namespace lsmf
+
+namespace lsmf {
+namespace {
+// cache(): zero the det F <= 0 counter bad_dev, run the cache kernel, read
+// the count back (synchronising the stream)
+lsmf_status cache_with_counter(lsmf_op *op, int *bad_dev, hipStream_t s, const char *what) {
+  if (!op) return fail(LSMF_ERR_INVALID_ARGUMENT, std::string(what) + ": null operator");
+  if (!op->material_set) return fail(LSMF_ERR_INVALID_ARGUMENT, std::string(what) + ": set_material was not called");
+  if (!op->linearization_set)
+    return fail(LSMF_ERR_INVALID_ARGUMENT, std::string(what) + ": set_linearization was not called");
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, bad_dev);
+  int rc = launch_table(op->dim, op->N)->cache(op, bad_dev, s);
+  if (rc) return fail(rc, std::string(what) + ": kernel launch failed");
+  int bad = 0;
+  LSMF_HIP_TRY(hipMemcpyAsync(&bad, bad_dev, sizeof(int), hipMemcpyDeviceToHost, s), "cache: read det F count");
+  LSMF_HIP_TRY(hipStreamSynchronize(s), "cache: synchronize");
+  if (bad > 0) {
+    op->cached = false;
+    return fail(LSMF_ERR_NONPOSITIVE_DET_F,
+                "det_F is not positive at " + std::to_string(bad) + " quadrature points");
+  }
+  op->cached = true;
+  op->diag_ok = false;
+  return LSMF_OK;
+}
+
+// Scratch of lsmf_cache_noalloc: a file-scope device array (each GPU holds
+// its own copy for the life of the process), reached through the device
+// address hipGetSymbolAddress returns for the current device -- in host
+// code the array's name is only a stand-in symbol.  kNoAllocScratchInts
+// ints: [0] counts the quadrature points with det F <= 0.
+constexpr int kNoAllocScratchInts = 64;
+__device__ __align__(256) int scratch[kNoAllocScratchInts];
+
+std::mutex g_scratch_mu;
+std::vector<int *> g_scratch_ptr;  // per device ordinal, nullptr until looked up
+
+lsmf_status scratch_for_current_device(int **out) {
+  int dev = -1;
+  LSMF_HIP_TRY(hipGetDevice(&dev), "cache_noalloc: current device");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    LSMF_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "cache_noalloc: scratch address");
+    if (!p) return fail(LSMF_ERR_HIP, "cache_noalloc: null scratch address");
+    g_scratch_ptr[dev] = static_cast<int *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return LSMF_OK;
+}
 }  // namespace
 }  // namespace lsmf
 
@@ -712,23 +766,14 @@ lsmf_status lsmf_set_linearization(lsmf_op *op, const double *u_dev, lsmf_stream
 
 lsmf_status lsmf_cache(lsmf_op *op, lsmf_stream stream) {
   if (!op) return fail(LSMF_ERR_INVALID_ARGUMENT, "cache: null operator");
-  if (!op->material_set) return fail(LSMF_ERR_INVALID_ARGUMENT, "cache: set_material was not called");
-  if (!op->linearization_set) return fail(LSMF_ERR_INVALID_ARGUMENT, "cache: set_linearization was not called");
-  hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, op->d_bad);
-  int rc = launch_table(op->dim, op->N)->cache(op, s);
-  if (rc) return fail(rc, "cache: kernel launch failed");
-  int bad = 0;
-  LSMF_HIP_TRY(hipMemcpyAsync(&bad, op->d_bad, sizeof(int), hipMemcpyDeviceToHost, s), "cache: read det F count");
-  LSMF_HIP_TRY(hipStreamSynchronize(s), "cache: synchronize");
-  if (bad > 0) {
-    op->cached = false;
-    return fail(LSMF_ERR_NONPOSITIVE_DET_F,
-                "det_F is not positive at " + std::to_string(bad) + " quadrature points");
-  }
-  op->cached = true;
-  op->diag_ok = false;
-  return LSMF_OK;
+  return cache_with_counter(op, op->d_bad, (hipStream_t)stream, "cache");
+}
+
+lsmf_status lsmf_cache_noalloc(lsmf_op *op, lsmf_stream stream) {
+  int *scr = nullptr;
+  const lsmf_status st = scratch_for_current_device(&scr);
+  if (st != LSMF_OK) return st;
+  return cache_with_counter(op, scr, (hipStream_t)stream, "cache_noalloc");
 }
 
 lsmf_status lsmf_compute_diagonal(lsmf_op *op, lsmf_stream stream) {

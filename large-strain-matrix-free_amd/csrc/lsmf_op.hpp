@@ -107,7 +107,7 @@ size_t elem_size(const lsmf_op *op);
 struct LaunchTable {
   int (*vmult)(const lsmf_op *, void *dst, const void *src, bool add, bool constr, unsigned ph, hipStream_t);
   int (*vmult_batched)(const lsmf_op *, void *dst, const void *src, int nvec, hipStream_t);
-  int (*cache)(const lsmf_op *, hipStream_t);
+  int (*cache)(const lsmf_op *, int *bad, hipStream_t);  // bad: det F <= 0 counter (zeroed by the caller)
   int (*diagonal)(const lsmf_op *, hipStream_t);
   int (*residual)(const lsmf_op *, double *rhs, hipStream_t);
   int (*residual_vjp)(const lsmf_op *, double *du, const double *w, hipStream_t);

@@ -91,6 +91,7 @@ _sig = {
     "lsmf_set_linearization": (_i, [_vp, _vp, _vp]),
     "lsmf_set_formulation": (_i, [_vp, _i]),
     "lsmf_cache": (_i, [_vp, _vp]),
+    "lsmf_cache_noalloc": (_i, [_vp, _vp]),
     "lsmf_compute_diagonal": (_i, [_vp, _vp]),
     "lsmf_vmult": (_i, [_vp, _vp, _vp, _i, ctypes.c_uint, _vp]),
     "lsmf_vmult_batched": (_i, [_vp, _vp, _vp, _i, _vp]),
