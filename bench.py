@@ -17,9 +17,9 @@ doc/large_strain.tex:739-753, divided by the kernel's event-timed duration)
 and a CPU baseline timed on the host.  Beside the headline it reports
   * the other caching variants and precisions on the same 128^3 mesh, each
     against its roofline (scalar also against the FP64 VALU peak),
-  * the cells layout (the path for meshes without the structured numbering:
-    the same mesh with its nodes renumbered at random, and in lexicographic
-    order),
+  * renumbered meshes (the reference's DoF renumbering): the cells layout
+    on the lexicographic and on a randomly renumbered mesh, and the tiles
+    the operator recovers from the cells' face neighbours on the latter,
   * the FP32-V-cycle against the all-FP64 Newton solve (3 load steps, CG to
     1e-8) on the 128^3 Q2 and the 64^3 Q4 meshes,
   * the 64^3 Q4 tensor2 configuration's vmult and roofline,
@@ -239,7 +239,7 @@ def vmult_record(mesh, mu, lam, ud, cons, caching, precision, steps, warmup, lay
     import lsmf
     dev = ud.device
     op = lsmf.NeoHookOperator(mesh, precision=precision, caching=caching, constrained=cons, layout=layout,
-                              use_mesh_colouring=layout == "auto")
+                              use_mesh_colouring=layout == "auto" and getattr(mesh, "cell_colour", None) is not None)
     op.set_material(mu, lam)
     op.set_linearization(ud)
     op.cache()
@@ -278,16 +278,18 @@ def vmult_record(mesh, mu, lam, ud, cons, caching, precision, steps, warmup, lay
 
 
 def permuted_mesh(mesh, seed):
-    """the same mesh with its nodes numbered in a seeded random order (no
-    structured numbering: the operator uses the cells layout)"""
+    """the same mesh with its nodes and its cells in a seeded random order
+    (no lexicographic numbering left)"""
     import numpy as np
     from types import SimpleNamespace
-    perm = np.random.default_rng(seed).permutation(mesh.n_nodes).astype(np.int32)
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(mesh.n_nodes).astype(np.int32)
+    order = rng.permutation(mesh.n_cells)
     coords = np.empty_like(mesh.coords)
     coords[perm] = mesh.coords
     return SimpleNamespace(dim=mesh.dim, degree=mesh.degree, n_cells=mesh.n_cells, n_nodes=mesh.n_nodes,
-                           n_dofs=mesh.n_dofs, cell_nodes=perm[mesh.cell_nodes], coords=coords,
-                           cell_colour=mesh.cell_colour), perm
+                           n_dofs=mesh.n_dofs, cell_nodes=np.ascontiguousarray(perm[mesh.cell_nodes[order]]),
+                           coords=coords, cell_colour=None), perm, order
 
 
 def variant_sweep(cfg_id, reps):
@@ -505,20 +507,23 @@ def main():
             rec = vmult_record(mesh, mu, lam, ud, cons, c2, p2, 10, 2, cfg_id=args.config, flops_per_dof=flops)
             progress(f"  {rec['ms']} ms, frac {rec['roofline']['frac']}")
             variants_128["records"].append(rec)
-        progress("cells layout: lexicographic numbering and a random node renumbering")
-        cells = {"workload": f"config {args.config} mesh, {caching} {args.precision}, one cell per lane group, "
-                             f"greedy cell colouring", "records": []}
+        progress("renumbered meshes: the cells layout and tiles recovered from the face topology")
+        cells = {"workload": f"config {args.config} mesh, {caching} {args.precision}; cells layout = one cell per "
+                             f"lane group, greedy cell colouring; auto = tiles recovered from the cells' face "
+                             f"neighbours whatever the numbering", "records": []}
         rec = vmult_record(mesh, mu, lam, ud, cons, caching, args.precision, 10, 2, layout="cells")
         rec["numbering"] = "lexicographic"
         cells["records"].append(rec)
-        pm, perm = permuted_mesh(mesh, 7)
+        pm, perm, order = permuted_mesh(mesh, 7)
         pcons = np.empty_like(cons)
         pcons.reshape(-1, mesh.dim)[perm] = cons.reshape(-1, mesh.dim)
         upd = torch.empty_like(ud)
         upd.view(-1, mesh.dim)[torch.tensor(perm, device=dev, dtype=torch.int64)] = ud.view(-1, mesh.dim)
-        rec = vmult_record(pm, mu, lam, upd, pcons, caching, args.precision, 5, 1, layout="cells")
-        rec["numbering"] = "random permutation of the nodes (seed 7)"
-        cells["records"].append(rec)
+        for layout, steps in (("cells", 5), ("auto", 10)):
+            rec = vmult_record(pm, mu[order], lam[order], upd, pcons, caching, args.precision, steps, 1,
+                               layout=layout)
+            rec["numbering"] = "random permutation of the nodes and of the cells (seed 7)"
+            cells["records"].append(rec)
         del pm, perm, pcons, upd
         progress(f"  {[r['ms'] for r in cells['records']]} ms")
     del ud

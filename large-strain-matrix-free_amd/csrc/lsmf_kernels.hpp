@@ -37,13 +37,16 @@ constexpr int kWG = 256;
 
 // Tiled-layout kinds (the TILE template argument of the cell kernels):
 // 0 = cells layout, 1 = tiles of one 256-thread workgroup, 2 = wide tiles of
-// one 512-thread workgroup.  3D Q4 operators with tensor2 or tensor4 caching
-// use wide tiles: four 125-point cells, 4x1x1, gather 425 nodes for the 256
-// they own (1.66 touches per node, 17-node rows of 408 bytes in FP64) where
-// 2x1x1 tiles gather 225 for 128 (1.76 touches, 9-node rows of 216 bytes):
-// 64^3 Q4 tensor2 FP64 1.73 -> 1.57 ms, tensor4 2.65 -> 2.52 ms.  Variants
-// with larger register footprints (scalar, tensor4_ns, scalar_referential:
-// 150-230 VGPRs) would run one 512-thread workgroup per CU and keep 256.
+// one 512-thread workgroup.  3D Q2 and Q4 operators with tensor2 or tensor4
+// caching use wide tiles.  Q4: four 125-point cells, 4x1x1, gather 425
+// nodes for the 256 they own (1.66 touches per node, 17-node rows of 408
+// bytes in FP64) where 2x1x1 tiles gather 225 for 128 (1.76 touches, 9-node
+// rows of 216 bytes): 64^3 Q4 tensor2 FP64 1.73 -> 1.57 ms, tensor4 2.65 ->
+// 2.52 ms.  Q2: 9x2x1 cells (285 nodes for 144 owned, 1.98 touches, rows of
+// 19 nodes as in 9x1x1): 128^3 tensor4 FP64 3.94 -> 3.87 ms, FP32 2.08 ->
+// 2.01 ms.  Variants with larger register footprints (scalar, tensor4_ns,
+// scalar_referential: 150-230 VGPRs in Q4) would run one 512-thread
+// workgroup per CU and keep 256 threads.
 constexpr int kTileWide = 2;
 template <int TILE> __host__ __device__ constexpr int tile_threads() { return TILE == kTileWide ? 2 * kWG : kWG; }
 constexpr uint32_t kNodeMask = (1u << 28) - 1u;
@@ -1100,7 +1103,10 @@ __device__ __forceinline__ void kinematics_collocation(const VmultParams<T> &P, 
 //        (fixed order) before one read-modify-write per node
 // ------------------------------------------------------------------------
 template <int DIM, int N, typename T, int VAR, bool ADD, bool CONSTR, unsigned PH, int TILE>
-__global__ void __launch_bounds__(tile_threads<TILE>()) k_vmult(const VmultParams<T> P) {
+// wide tiles: at least 4 waves per SIMD (two 512-thread workgroups per CU,
+// at most 128 VGPRs; the second argument is HIP's minimum waves per EU).
+// Six (80 VGPRs) spill the tensor2 FP64 kernel: 64^3 Q4 1.57 -> 1.74 ms.
+__global__ void __launch_bounds__(tile_threads<TILE>(), TILE == kTileWide ? 4 : 1) k_vmult(const VmultParams<T> P) {
   using CS = CellShape<DIM, N>;
   // threads and cells of this workgroup: a tile, or CS::CPW cells
   constexpr int WG = tile_threads<TILE>();

@@ -128,6 +128,111 @@ bool structured_dims(const lsmf_mesh_desc *m, int dim, int p, int nqp, long long
   return true;
 }
 
+// Cell grid of a logically structured mesh: cell_at[cx + n0 (cy + n1 cz)]
+// is the mesh cell at grid position (cx, cy, cz), recovered from the cells'
+// shared faces whatever the node and cell numbering (a box renumbered by
+// deal.II's Cuthill-McKee, include/mf_elasticity.h:1610-1708, tiles like
+// one in lsmf_mesh_create's lexicographic numbering, which is checked
+// first without any search).  Every cell must be in standard orientation:
+// the neighbour across local face +d has this cell across its local face
+// -d.  Returns false for anything else (the cells layout then serves it).
+bool cell_grid(const lsmf_mesh_desc *m, int dim, int p, int nqp, long long n[3], std::vector<int32_t> &cell_at) {
+  const int nc = m->n_cells;
+  if (structured_dims(m, dim, p, nqp, n)) {
+    cell_at.resize(nc);
+    for (int c = 0; c < nc; ++c) cell_at[c] = c;
+    return true;
+  }
+  const int N = p + 1, nf = 2 * dim, ncorner = 1 << (dim - 1);
+  // local index of corner k of face f = 2 d + s
+  auto corner = [&](int f, int k) {
+    const int d = f / 2, s = f % 2;
+    int bits[3] = {0, 0, 0}, b = 0;
+    for (int e = 0; e < dim; ++e) bits[e] = e == d ? s : (k >> b++) & 1;
+    int a = 0, stride = 1;
+    for (int e = 0; e < dim; ++e, stride *= N) a += bits[e] * p * stride;
+    return a;
+  };
+  struct Face {
+    int32_t v[4];
+    int64_t id;  // cell * nf + face
+  };
+  std::vector<Face> faces((size_t)nc * nf);
+  for (int c = 0; c < nc; ++c)
+    for (int f = 0; f < nf; ++f) {
+      Face &F = faces[(size_t)c * nf + f];
+      F.v[2] = F.v[3] = -1;
+      for (int k = 0; k < ncorner; ++k) F.v[k] = m->cell_nodes[(size_t)c * nqp + corner(f, k)];
+      std::sort(F.v, F.v + ncorner);
+      F.id = (int64_t)c * nf + f;
+    }
+  std::sort(faces.begin(), faces.end(), [](const Face &a, const Face &b) {
+    return std::lexicographical_compare(a.v, a.v + 4, b.v, b.v + 4);
+  });
+  std::vector<int32_t> nbr((size_t)nc * nf, -1);
+  for (size_t i = 0; i < faces.size();) {
+    size_t j = i + 1;
+    while (j < faces.size() && std::equal(faces[i].v, faces[i].v + 4, faces[j].v)) ++j;
+    if (j - i > 2) return false;
+    if (j - i == 2) {
+      const int64_t a = faces[i].id, b = faces[i + 1].id;
+      const int fa = (int)(a % nf), fb = (int)(b % nf);
+      if ((fa ^ 1) != fb) return false;  // not face +d against face -d
+      nbr[a] = (int32_t)(b / nf);
+      nbr[b] = (int32_t)(a / nf);
+    }
+    i = j;
+  }
+  // the one cell without -x, -y, -z neighbours is the grid's origin
+  int origin = -1;
+  for (int c = 0; c < nc; ++c) {
+    bool o = true;
+    for (int d = 0; d < dim; ++d) o = o && nbr[(size_t)c * nf + 2 * d] < 0;
+    if (o) {
+      if (origin >= 0) return false;
+      origin = c;
+    }
+  }
+  if (origin < 0) return false;
+  n[0] = n[1] = n[2] = 1;
+  for (int d = 0; d < dim; ++d) {
+    long long len = 1;
+    for (int c = origin; nbr[(size_t)c * nf + 2 * d + 1] >= 0; c = nbr[(size_t)c * nf + 2 * d + 1])
+      if (++len > nc) return false;
+    n[d] = len;
+  }
+  if (n[0] * n[1] * n[2] != nc) return false;
+  cell_at.assign(nc, -1);
+  std::vector<uint8_t> seen(nc, 0);
+  auto at = [&](long long x, long long y, long long z) -> int32_t & { return cell_at[x + n[0] * (y + n[1] * z)]; };
+  for (long long z = 0; z < n[2]; ++z)
+    for (long long y = 0; y < n[1]; ++y)
+      for (long long x = 0; x < n[0]; ++x) {
+        int32_t c = x > 0   ? nbr[(size_t)at(x - 1, y, z) * nf + 1]
+                    : y > 0 ? nbr[(size_t)at(x, y - 1, z) * nf + 3]
+                    : z > 0 ? nbr[(size_t)at(x, y, z - 1) * nf + 5]
+                            : origin;
+        if (c < 0 || seen[c]) return false;
+        seen[c] = 1;
+        at(x, y, z) = c;
+      }
+  // every face link agrees with the grid
+  for (long long z = 0; z < n[2]; ++z)
+    for (long long y = 0; y < n[1]; ++y)
+      for (long long x = 0; x < n[0]; ++x) {
+        const long long g[3] = {x, y, z};
+        const int32_t c = at(x, y, z);
+        for (int d = 0; d < dim; ++d)
+          for (int s = 0; s < 2; ++s) {
+            long long h[3] = {g[0], g[1], g[2]};
+            h[d] += s ? 1 : -1;
+            const int32_t expect = (h[d] < 0 || h[d] >= n[d]) ? -1 : at(h[0], h[1], h[2]);
+            if (nbr[(size_t)c * nf + 2 * d + s] != expect) return false;
+          }
+      }
+  return true;
+}
+
 // Some support point lies off its cell's Q1 (multi)linear map of the
 // vertices (beyond round-off relative to the cell size): the current
 // configuration then needs the Eulerian Q_p metric of all support points.
@@ -195,16 +300,20 @@ bool tile_dims(int dim, int N, int kind, TileDims &t) {
   return true;
 }
 
-// Tiles of a structured mesh, coloured by the parity of their position so
-// that no two tiles of one colour share a node.  Fills perm (tile order, the
-// cells of a tile consecutive in slot order), tile_ptr, the tile node table
-// (first-touch bit: the node's first tile in colour order) and tile info.
-int build_tiles(lsmf_op *op, const lsmf_mesh_desc *mesh, const long long n[3], std::vector<uint32_t> &tnodes,
-                std::vector<int32_t> &tinfo) {
-  const int dim = op->dim, p = op->degree;
+// Tiles of a logically structured mesh (cell grid cell_at, n cells per
+// direction), coloured by the parity of their position so that no two tiles
+// of one colour share a node.  Fills perm (tile order, the cells of a tile
+// consecutive in slot order), tile_ptr, the tile node table (node ids from
+// the cells' own node lists; first-touch bit: the node's first tile in
+// colour order) and tile info.  Returns kNotTileable when cells that share
+// a tile node position disagree about its node (the cells layout then
+// serves the mesh).
+constexpr int kNotTileable = 1;
+int build_tiles(lsmf_op *op, const lsmf_mesh_desc *mesh, const long long n[3], const std::vector<int32_t> &cell_at,
+                std::vector<uint32_t> &tnodes, std::vector<int32_t> &tinfo) {
+  const int dim = op->dim, p = op->degree, N = p + 1, nqp = op->nqp;
   TileDims td;
   if (!tile_dims(dim, op->N, op->tile_kind, td)) return fail(LSMF_ERR_UNSUPPORTED, "tiles: no tile shape");
-  const long long Nx = n[0] * p + 1, Ny = n[1] * p + 1, Nz = dim == 3 ? n[2] * p + 1 : 1;
   const long long TXn = (n[0] + td.tx - 1) / td.tx, TYn = (n[1] + td.ty - 1) / td.ty,
                   TZn = (n[2] + td.tz - 1) / td.tz;
   const long long n_tiles = TXn * TYn * TZn;
@@ -234,20 +343,35 @@ int build_tiles(lsmf_op *op, const lsmf_mesh_desc *mesh, const long long n[3], s
       const int ez = (int)std::min<long long>(td.tz, n[2] - bz * td.tz);
       tinfo[(size_t)ti * 2] = pos;
       tinfo[(size_t)ti * 2 + 1] = ex | (ey << 8) | (ez << 16);
+      auto tile_cell = [&](int sx, int sy, int sz) {
+        return cell_at[(bx * td.tx + sx) + n[0] * ((by * td.ty + sy) + n[1] * (bz * td.tz + sz))];
+      };
       for (int sz = 0; sz < ez; ++sz)
         for (int sy = 0; sy < ey; ++sy)
           for (int sx = 0; sx < ex; ++sx) {
-            const long long cell = (bx * td.tx + sx) + n[0] * ((by * td.ty + sy) + n[1] * (bz * td.tz + sz));
-            op->perm[pos] = (int)cell;
+            const int cell = tile_cell(sx, sy, sz);
+            op->perm[pos] = cell;
             op->inv_perm[cell] = pos;
             ++pos;
           }
       for (int l = 0; l < td.tn; ++l) {
-        const long long X = bx * td.tx * p + l % td.nx, Y = by * td.ty * p + (l / td.nx) % td.ny,
-                        Z = bz * td.tz * p + l / (td.nx * td.ny);
-        if (X >= Nx || Y >= Ny || Z >= Nz) continue;
-        const long long node = X + Nx * (Y + Ny * Z);
-        if (stamp[node] == col) return fail(LSMF_ERR_INVALID_ARGUMENT, "tiles: two tiles of one colour share a node");
+        const int X = l % td.nx, Y = (l / td.nx) % td.ny, Z = l / (td.nx * td.ny);
+        if (X > ex * p || Y > ey * p || Z > ez * p) continue;  // beyond a ragged tile
+        // the node from every cell of the tile that holds this position
+        // (the kernels sum those cells' contributions into it)
+        const int xs[2] = {X == 0 ? 0 : (X - 1) / p, std::min(X / p, ex - 1)};
+        const int ys[2] = {Y == 0 ? 0 : (Y - 1) / p, std::min(Y / p, ey - 1)};
+        const int zs[2] = {Z == 0 ? 0 : (Z - 1) / p, std::min(Z / p, ez - 1)};
+        long long node = -1;
+        for (int cz = zs[0]; cz <= zs[1]; ++cz)
+          for (int cy = ys[0]; cy <= ys[1]; ++cy)
+            for (int cx = xs[0]; cx <= xs[1]; ++cx) {
+              const int a = (X - cx * p) + N * ((Y - cy * p) + (dim == 3 ? N * (Z - cz * p) : 0));
+              const long long nd = mesh->cell_nodes[(size_t)tile_cell(cx, cy, cz) * nqp + a];
+              if (node >= 0 && nd != node) return kNotTileable;
+              node = nd;
+            }
+        if (stamp[node] == col) return kNotTileable;  // two tiles of one colour share a node
         stamp[node] = col;
         min_col[node] = std::min(min_col[node], col);
         tnodes[(size_t)ti * td.tn + l] = (uint32_t)node;
@@ -487,19 +611,26 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   std::vector<uint32_t> tnodes;
   std::vector<int32_t> tinfo;
   long long sn[3] = {1, 1, 1};
-  op->tiled = opt->layout == LSMF_LAYOUT_AUTO && nnc == 0 && structured_dims(mesh, dim, p, nqp, sn);
-  // wide (512-thread, four-cell) tiles for 3D Q4 tensor2 / tensor4 products
-#ifndef LSMF_WIDE_Q2
-#define LSMF_WIDE_Q2 0
-#endif
+  std::vector<int32_t> cell_at;
+  op->tiled = opt->layout == LSMF_LAYOUT_AUTO && nnc == 0 && cell_grid(mesh, dim, p, nqp, sn, cell_at);
+  // wide (512-thread) tiles for 3D Q2 / Q4 tensor2 and tensor4 products
   op->tile_kind = !op->tiled ? 0
-                  : (dim == 3 && (op->N == 5 || (op->N == 3 && LSMF_WIDE_Q2)) &&
+                  : (dim == 3 && (op->N == 5 || op->N == 3) &&
                      (op->caching == LSMF_CACHING_TENSOR2 || op->caching == LSMF_CACHING_TENSOR4))
                       ? kTileWide
                       : 1;
   if (op->tiled) {
-    const int rc = build_tiles(op, mesh, sn, tnodes, tinfo);
-    if (rc) {
+    const std::vector<int> perm0 = op->perm, inv0 = op->inv_perm;
+    const int rc = build_tiles(op, mesh, sn, cell_at, tnodes, tinfo);
+    if (rc == kNotTileable) {  // the colour order of the cells layout stays
+      op->tiled = false;
+      op->tile_kind = 0;
+      op->perm = perm0;
+      op->inv_perm = inv0;
+      op->tile_ptr.clear();
+      tnodes.clear();
+      tinfo.clear();
+    } else if (rc) {
       delete op;
       return rc;
     }

@@ -10,7 +10,8 @@
     a scalar loss
   * gradient of a face-misfit objective through a Newton solve (one adjoint
     GMG-CG solve + one VJP) against differences of two full solves, also
-    with Q4 tensor2 and FP32 multigrid levels
+    with Q4 tensor2 and FP32 multigrid levels, and on the 48^3 Q2
+    configuration at its full size
 """
 import numpy as np
 import pytest
@@ -201,6 +202,58 @@ def test_misfit_gradient_through_newton_solve(degree, caching, mg_precision, sig
             _, us = solve(mu + sgn * h * b, lam + sgn * h * c)
             dofs = (face[:, None] * 3 + np.arange(3)).reshape(-1)
             dofs = dofs[cons[dofs] == 0]
+            d = us[dofs] - target[dofs]
+            Js.append(0.5 * float(d @ d))
+        fd = (Js[0] - Js[1]) / (2 * h)
+        an = float(gmu @ b + gla @ c)
+        log(f"  direction {k}: FD {fd:.8e} adjoint {an:.8e}")
+        assert abs(fd - an) < 1e-5 * abs(an)
+
+
+def test_config3_misfit_gradient_at_full_size(device):
+    """the differentiation workload at its size: 48^3 Q2 (2,738,019 DoF),
+    log-normal mu, lambda (sigma = 0.5), x=0 clamped, u_x = 0.01 at x=1 in
+    one load step, FP64 levels; J = misfit of the x=1 face displacement
+    against the solution for a second seeded field.  dJ/dmu, dJ/dlambda
+    from one adjoint solve + one VJP against central differences of two
+    full Newton solves (h = 1e-5) on 2 seeded directions, to 1e-5
+    relative"""
+    cfg = lsmf.CONFIGS[3]
+    n = cfg["n"]
+    log(f"[adjoint solve] {n}^3 Q2 (config 3 at full size)")
+    mesh = lsmf.Mesh(3, 2, [n] * 3)
+    assert mesh.n_dofs == 2738019
+    cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
+    o = lsmf.solver_options(dim=3, degree=2, n_q_1d=3, n_cells_coarse=[3] * 3, n_levels=5, caching="tensor4",
+                            mg_caching="tensor4", mg_precision="fp64", coarse_solver="direct", tol_lin=1e-12,
+                            tol_f=1e-13, tol_u=1e-12, dt=1.0, t_end=1.0,
+                            dirichlet={0: ([0, 1, 2], [0.0] * 3), 1: ([0], [0.01])})
+
+    def solve(mu, lam):
+        S = lsmf.Solver(o, mu, lam)
+        S.run()
+        return S, S.solution()
+
+    seed = lsmf.workload.SEED
+    mu_t, lam_t = lsmf.lognormal_material(mesh.n_cells, 1.0, cfg["nu"], cfg["sigma"], seed + 31)
+    _, target = solve(mu_t, lam_t)
+    mu, lam = lsmf.lognormal_material(mesh.n_cells, 1.0, cfg["nu"], cfg["sigma"], seed + 3)
+    S, u_star = solve(mu, lam)
+    face = mesh.boundary_nodes(1)
+    J, gmu, gla = misfit_gradient(S, mesh, u_star, mu, lam, target, face, cons)
+    del S
+    assert J > 0
+    dofs = (face[:, None] * 3 + np.arange(3)).reshape(-1)
+    dofs = dofs[cons[dofs] == 0]
+    rng = np.random.default_rng(48)
+    for k in range(2):
+        b = rng.standard_normal(mesh.n_cells) * mu
+        c = rng.standard_normal(mesh.n_cells) * lam
+        h = 1e-5
+        Js = []
+        for sgn in (1.0, -1.0):
+            S2, us = solve(mu + sgn * h * b, lam + sgn * h * c)
+            del S2
             d = us[dofs] - target[dofs]
             Js.append(0.5 * float(d @ d))
         fd = (Js[0] - Js[1]) / (2 * h)

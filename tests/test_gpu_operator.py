@@ -433,3 +433,64 @@ def test_batched_vmult_equals_single_products(dim, p, n, layout, caching, precis
         assert lsmf._lib.lsmf_vmult_batched(op._h, ctypes.c_void_p(X[1].data_ptr()),
                                             ctypes.c_void_p(X.data_ptr()), 2, None) == -1
         assert b"overlap" in lsmf._lib.lsmf_last_error()
+
+
+def renumbered(mesh, seed, rotate_cells=()):
+    """the same mesh with its nodes and cells in a seeded random order (the
+    reference's DoF renumbering); cells in `rotate_cells` list their nodes
+    turned by 180 degrees about z (local (a0, a1) -> (p - a0, p - a1)), a
+    valid cell that is not in standard orientation"""
+    from types import SimpleNamespace
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(mesh.n_nodes).astype(np.int32)
+    order = rng.permutation(mesh.n_cells)
+    coords = np.empty_like(mesh.coords)
+    coords[perm] = mesh.coords
+    cn = perm[mesh.cell_nodes[order]]
+    N = mesh.degree + 1
+    loc = np.arange(N ** mesh.dim)
+    a0, a1, rest = loc % N, (loc // N) % N, loc // (N * N)
+    turned = (N - 1 - a0) + N * (N - 1 - a1) + N * N * rest
+    for c in rotate_cells:
+        cn[c] = cn[c][turned]
+    return SimpleNamespace(dim=mesh.dim, degree=mesh.degree, n_cells=mesh.n_cells, n_nodes=mesh.n_nodes,
+                           n_dofs=mesh.n_dofs, cell_nodes=np.ascontiguousarray(cn), coords=coords,
+                           cell_colour=None), perm, order
+
+
+@pytest.mark.parametrize("dim,p,n,rotate", [(3, 2, 5, False), (3, 4, 3, False), (2, 2, 7, False), (3, 1, 4, False),
+                                             (3, 2, 4, True), (2, 4, 3, True)])
+@pytest.mark.parametrize("caching", ["tensor4", "tensor2", "scalar"])
+def test_renumbered_mesh_gives_the_same_operator(dim, p, n, rotate, caching, device):
+    """a structured mesh with its nodes and cells numbered at random is
+    still recognised as logically structured (tiles from the cells' face
+    neighbours) and gives the lexicographic mesh's operator; with one cell
+    not in standard orientation it is served by the cells layout: vmult,
+    diagonal and residual agree with the lexicographic mesh to 1e-13"""
+    import torch
+    log(f"[renumbered] dim={dim} p={p} n={n} rotate={rotate} {caching}")
+    mesh, mu, lam, u, cons = problem(dim, p, n)
+    rm, perm, order = renumbered(mesh, 11, rotate_cells=(3,) if rotate else ())
+    pu = np.empty_like(u)
+    pu.reshape(-1, dim)[perm] = u.reshape(-1, dim)
+    pcons = np.empty_like(cons)
+    pcons.reshape(-1, dim)[perm] = cons.reshape(-1, dim)
+    v = np.random.default_rng(3).random(mesh.n_dofs)
+    pv = np.empty_like(v)
+    pv.reshape(-1, dim)[perm] = v.reshape(-1, dim)
+    out = []
+    for m, mu_, lam_, uu, cc, vv in ((mesh, mu, lam, u, cons, v), (rm, mu[order], lam[order], pu, pcons, pv)):
+        op = lsmf.NeoHookOperator(m, precision="fp64", caching=caching, constrained=cc,
+                                  use_mesh_colouring=m is mesh)
+        op.set_material(mu_, lam_)
+        op.set_linearization(torch.tensor(uu, device=device))
+        op.cache()
+        y = torch.full((m.n_dofs,), float("nan"), dtype=torch.float64, device=device)
+        op.vmult(y, torch.tensor(vv, device=device))
+        op.compute_diagonal()
+        r = torch.full_like(y, float("nan"))
+        op.residual(r)
+        out.append([y.cpu().numpy(), op.diagonal().cpu().numpy(), r.cpu().numpy()])
+    for k, name in enumerate(("vmult", "diagonal", "residual")):
+        back = out[1][k].reshape(-1, dim)[perm].reshape(-1)  # renumbered -> lexicographic
+        assert rel(back, out[0][k]) < 1e-13, name
