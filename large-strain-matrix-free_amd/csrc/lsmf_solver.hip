@@ -214,7 +214,7 @@ static void band_launch(const DevBand &b, int d, const int *dims, const T *in, T
   int od[3] = {dims[0], dims[1], dims[2]};
   od[d] = b.n_out;
   if (od[1] > 65535 || od[2] > 65535) throw HipError(LSMF_ERR_UNSUPPORTED, "transfer: more than 65535 nodes per direction");
-  const dim3 grid((unsigned)((od[0] * C + 255) / 256), (unsigned)od[1], (unsigned)od[2]);
+  const dim3 grid((unsigned)((od[0] + 255) / 256), (unsigned)od[1], (unsigned)od[2]);
   const T *wt = b.weights<T>();
   const int a = add ? 1 : 0;
   if (d == 0)

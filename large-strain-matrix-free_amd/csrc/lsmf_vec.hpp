@@ -330,34 +330,57 @@ __global__ void __launch_bounds__(256) k_bgj_update(double *__restrict__ A, int 
 // (n2, n1, n0, C) (x fastest, C components interleaved); the output has
 // n_DIR replaced by n_out:
 //   out[.., o, ..] = sum_{k < len[o]} w[o * W + k] in[.., start[o] + k, ..]
-// (add != 0: out +=, the prolongate-and-add of the V-cycle).  Grid: x
-// covers the m0 * C outputs of one x-line, 256 per block; y and z are the
-// output's i1 and i2, so no index is divided but by the constant C.  Along
-// y and z the row o is the same for the whole block (its start, length and
-// weights are scalar loads) and the taps loop over its nonzero length;
-// along x the lanes differ in o and loop over the padded width W (zero
-// weights past a row's length).  T is the level's precision: FP32 levels
-// transfer in FP32 with FP32 weights.
+// (add != 0: out +=, the prolongate-and-add of the V-cycle).  One thread
+// per output node: its C components come in and go out as one access per
+// tap (12 bytes in FP32, 24 in FP64), and each tap's weight is loaded once
+// for the C components -- a thread per component loaded every weight C
+// times and every value with its own narrow load, and was bound by the
+// number of load instructions, not by HBM.  Grid: x covers the m0 nodes of
+// one x-line, 256 per block; y and z are the output's i1 and i2 (no index
+// division).  Along y and z the row o is the same for the whole block (its
+// start, length and weights are scalar loads) and the taps loop over its
+// nonzero length; along x the lanes differ in o and loop over the padded
+// width W (zero weights past a row's length).  T is the level's precision:
+// FP32 levels transfer in FP32 with FP32 weights.
+template <int C, typename T> struct BandNode {
+  T v[C];
+};
 template <typename T, int DIR, int C>
 __global__ void __launch_bounds__(256) k_band(const T *__restrict__ in, T *__restrict__ out,
                                               const int32_t *__restrict__ start, const int32_t *__restrict__ len,
                                               const T *__restrict__ w, int W, int n0, int n1, int n_out, int add) {
   const int m0 = DIR == 0 ? n_out : n0, m1 = DIR == 1 ? n_out : n1;
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= m0 * C) return;
-  const int i0 = e / C, c = e - i0 * C;
+  const int i0 = blockIdx.x * 256 + threadIdx.x;
+  if (i0 >= m0) return;
   const int o = DIR == 0 ? i0 : (DIR == 1 ? (int)blockIdx.y : (int)blockIdx.z);
   const int s = start[o];
   const int j0 = DIR == 0 ? s : i0, j1 = DIR == 1 ? s : (int)blockIdx.y, j2 = DIR == 2 ? s : (int)blockIdx.z;
   const size_t line = (size_t)n0 * C;
-  const T *src = in + ((size_t)j2 * n1 + j1) * line + (size_t)j0 * C + c;
+  const T *src = in + ((size_t)j2 * n1 + j1) * line + (size_t)j0 * C;
   const size_t stride = DIR == 0 ? (size_t)C : (DIR == 1 ? line : line * n1);
   const T *wr = w + (size_t)o * W;
   const int nk = DIR == 0 ? W : len[o];
-  T acc = T(0);
-  for (int k = 0; k < nk; ++k) acc = fma(wr[k], src[(size_t)k * stride], acc);
-  T *dst = out + ((size_t)blockIdx.z * m1 + blockIdx.y) * ((size_t)m0 * C) + e;
-  *dst = add ? *dst + acc : acc;
+  T acc[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) acc[c] = T(0);
+  for (int k = 0; k < nk; ++k) {
+    BandNode<C, T> x;
+    __builtin_memcpy(&x, src + (size_t)k * stride, sizeof(x));
+    const T wk = wr[k];
+#pragma unroll
+    for (int c = 0; c < C; ++c) acc[c] = fma(wk, x.v[c], acc[c]);
+  }
+  T *dst = out + (((size_t)blockIdx.z * m1 + blockIdx.y) * m0 + i0) * C;
+  BandNode<C, T> r;
+#pragma unroll
+  for (int c = 0; c < C; ++c) r.v[c] = acc[c];
+  if (add) {
+    BandNode<C, T> d;
+    __builtin_memcpy(&d, dst, sizeof(d));
+#pragma unroll
+    for (int c = 0; c < C; ++c) r.v[c] += d.v[c];
+  }
+  __builtin_memcpy(dst, &r, sizeof(r));
 }
 
 }  // namespace lsmf
