@@ -233,13 +233,14 @@ def timed_vmults(op, x, y, steps, warmup, torch):
 
 
 def vmult_record(mesh, mu, lam, ud, cons, caching, precision, steps, warmup, layout="auto", cfg_id=None,
-                 flops_per_dof=None):
+                 flops_per_dof=None, node_constraints=None):
     """one operator's vmult throughput and roofline on `mesh`"""
     import torch
     import lsmf
     dev = ud.device
     op = lsmf.NeoHookOperator(mesh, precision=precision, caching=caching, constrained=cons, layout=layout,
-                              use_mesh_colouring=layout == "auto" and getattr(mesh, "cell_colour", None) is not None)
+                              use_mesh_colouring=layout == "auto" and getattr(mesh, "cell_colour", None) is not None,
+                              node_constraints=node_constraints)
     op.set_material(mu, lam)
     op.set_linearization(ud)
     op.cache()
@@ -275,6 +276,40 @@ def vmult_record(mesh, mu, lam, ud, cons, caching, precision, steps, warmup, lay
     del op, x, y
     torch.cuda.empty_cache()
     return rec
+
+
+def hanging_record(caching, precision, n=64):
+    """vmult throughput with hanging-node constraints: the unit cube with
+    its half x > 1/2 refined once (lsmf.half_refined_box, Q2, n^3 coarse-cell
+    resolution), x = 0 clamped, against the same cells without the
+    constraints (the cells layout either way: the mesh is not one grid)"""
+    import numpy as np
+    import torch
+    import lsmf
+    mesh, nc = lsmf.half_refined_box(2, n)
+    x = mesh.coords
+    cons = np.zeros(mesh.n_dofs, np.uint8)
+    left = np.where(np.abs(x[:, 0]) < 1e-12)[0]
+    for c in range(3):
+        cons[left * 3 + c] = 1
+    rng = np.random.default_rng(lsmf.workload.SEED)
+    mu, lam = 1.0 + rng.random(mesh.n_cells), 1.5 + rng.random(mesh.n_cells)
+    # a displacement continuous across the interface: hanging values from their masters
+    u = (0.02 * np.sin(np.pi * (x @ np.arange(1, 4)))[:, None] * np.ones(3))
+    nodes, ptr, masters, w = nc
+    for k, h in enumerate(nodes):
+        u[h] = w[ptr[k]:ptr[k + 1]] @ u[masters[ptr[k]:ptr[k + 1]]]
+    ud = torch.tensor(u.reshape(-1), device="cuda:0")
+    out = {"workload": f"unit cube, half x > 1/2 refined once: {mesh.n_cells} Q2 cells, {mesh.n_dofs} DoF, "
+                       f"{len(nodes)} hanging nodes (3 or 9 masters each on the coarse faces), {caching} {precision}, "
+                       f"cells layout"}
+    out["with_constraints"] = vmult_record(mesh, mu, lam, ud, cons, caching, precision, 10, 2, layout="cells",
+                                           node_constraints=nc)
+    out["without_constraints"] = vmult_record(mesh, mu, lam, ud, cons, caching, precision, 10, 2, layout="cells")
+    out["overhead"] = round(out["with_constraints"]["ms"] / out["without_constraints"]["ms"] - 1.0, 4)
+    del ud
+    torch.cuda.empty_cache()
+    return out
 
 
 def permuted_mesh(mesh, seed):
@@ -533,7 +568,7 @@ def main():
     if not args.no_solve and "load_steps" in cfg:
         solve = solve_pair(args.config, args.n, mu, lam, args.solve_tol, args.coarse)
 
-    config4 = variants = adjoint = gradient = None
+    config4 = variants = adjoint = gradient = hanging = None
     if extras:
         if args.config != 4:
             progress("config 4: 64^3 Q4 tensor2")
@@ -554,6 +589,9 @@ def main():
             progress("config 4 adjoint time/memory trade (tensor2)")
             config4["adjoint"] = adjoint_trade(4, 10, "tensor2")
             del m4, mu4, lam4, cons4
+        progress("hanging-node constraints: half-refined cube, Q2 tensor4 FP64")
+        hanging = hanging_record("tensor4", "fp64")
+        progress(f"  {hanging['with_constraints']['ms']} ms with, {hanging['without_constraints']['ms']} without")
         progress("config 3: residual VJP trade and the misfit gradient through a Newton solve (48^3 Q2)")
         adjoint = adjoint_trade(3, 20)
         gradient = misfit_gradient_timing(3)
@@ -595,6 +633,7 @@ def main():
         "variants_same_mesh": variants_128,
         "cells_layout": cells,
         "config4": config4,
+        "hanging_nodes": hanging,
         "config3_adjoint": adjoint,
         "config3_misfit_gradient": gradient,
         "caching_variants_32": variants,
@@ -605,7 +644,7 @@ def main():
 # FP64 floating-point operations per DoF of one scalar-variant vmult
 # (the kernel's FP64 VALU adds, multiplies and fused multiply-adds counted
 # by rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 (FMA = 2), divided by the DoFs):
-# profiles/k_vmult_config2_scalar_fp64.md
+# profiles/flops_scalar_fp64.json
 SCALAR_FLOPS_PER_DOF = {}
 _sf = os.path.join(ROOT, "profiles", "flops_scalar_fp64.json")
 if os.path.exists(_sf):

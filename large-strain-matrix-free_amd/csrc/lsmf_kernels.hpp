@@ -1102,11 +1102,16 @@ __device__ __forceinline__ void kinematics_collocation(const VmultParams<T> &P, 
 //        gathered once into LDS and the cells' results are summed there
 //        (fixed order) before one read-modify-write per node
 // ------------------------------------------------------------------------
+// Minimum waves per SIMD of the fused cell kernel (HIP's second
+// __launch_bounds__ argument): wide tiles at least 4 (two 512-thread
+// workgroups per CU, at most 128 VGPRs); the wide tensor2 kernel 6 (three
+// workgroups, at most 80 VGPRs, which its FP64 form needs no more than).
+template <int VAR, int TILE> constexpr int vmult_min_waves() {
+  return TILE != kTileWide ? 1 : VAR == V_TENSOR2 ? 6 : 4;
+}
+
 template <int DIM, int N, typename T, int VAR, bool ADD, bool CONSTR, unsigned PH, int TILE>
-// wide tiles: at least 4 waves per SIMD (two 512-thread workgroups per CU,
-// at most 128 VGPRs; the second argument is HIP's minimum waves per EU).
-// Six (80 VGPRs) spill the tensor2 FP64 kernel: 64^3 Q4 1.57 -> 1.74 ms.
-__global__ void __launch_bounds__(tile_threads<TILE>(), TILE == kTileWide ? 4 : 1) k_vmult(const VmultParams<T> P) {
+__global__ void __launch_bounds__(tile_threads<TILE>(), (vmult_min_waves<VAR, TILE>())) k_vmult(const VmultParams<T> P) {
   using CS = CellShape<DIM, N>;
   // threads and cells of this workgroup: a tile, or CS::CPW cells
   constexpr int WG = tile_threads<TILE>();
@@ -1116,13 +1121,15 @@ __global__ void __launch_bounds__(tile_threads<TILE>(), TILE == kTileWide ? 4 : 
   constexpr bool RECOMP = (VAR == V_RESIDUAL || VAR == V_ADJOINT);  // no cache, c from ln J in FP64
   constexpr int NF = CacheFields<DIM, (RECOMP ? V_SCALAR : VAR)>::value;
   constexpr bool KIN = (VAR == V_SCALAR || VAR == V_SCALAR_REF || RECOMP);
-  constexpr int TABN = tab_size<N>(VAR);
+  // the 1D tables in LDS serve the 2D sum factorization and the kinematics;
+  // the 3D cached tangents read their line operators from P.sf
+  constexpr int TABN = (DIM == 2 || KIN) ? tab_size<N>(VAR) : 0;
   constexpr int RED = (VAR == V_ADJOINT) ? CPW * 2 * NQP : 0;  // per-cell reductions
   // per-cell LDS buffers: 3D sums in place (SFLine, NB arrays), 2D ping-pongs
   constexpr int SLOTW = (DIM == 3 ? 1 : 2) * NB * NQP;
   __shared__ T lds[CPW * SLOTW + TABN + (KIN ? CPW * NC * DIM : 0) + RED];
   __shared__ T tile_v[TILE ? TN * DIM : 1];         // tile nodes' source values
-  __shared__ uint32_t tile_c[TILE ? TN : 1];        // tile nodes' codes
+  __shared__ uint32_t tile_c[TILE && KIN ? TN : 1];  // tile nodes' codes (kinematics)
   T *tab = lds + CPW * SLOTW;
   T *Xall = tab + TABN;
   T *Rall = Xall + (KIN ? CPW * NC * DIM : 0);
@@ -1198,8 +1205,12 @@ __global__ void __launch_bounds__(tile_threads<TILE>(), TILE == kTileWide ? 4 : 
   const int ln = on ? (sx * (N - 1) + i0) + TS::NX * ((sy * (N - 1) + i1) + TS::NY * (sz * (N - 1) + i2)) : 0;
   if constexpr (TILE) {
     if (t < TN) {
-      tile_c[t] = tcode;
-      if constexpr (VAR != V_RESIDUAL) {
+      if constexpr (KIN) tile_c[t] = tcode;
+      if constexpr (VAR == V_RESIDUAL) {
+        // no source values: the tile holds the destination values throughout
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) tile_v[t * DIM + c] = dv[c];
+      } else {
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
           const bool zero = !tvalid || (CONSTR && ((tcode >> (kConsShift + c)) & 1u));
@@ -1208,8 +1219,10 @@ __global__ void __launch_bounds__(tile_threads<TILE>(), TILE == kTileWide ? 4 : 
       }
     }
     lds_barrier();
-    code = tile_c[ln];
-    node = code & kNodeMask;
+    if constexpr (KIN) {
+      code = tile_c[ln];
+      node = code & kNodeMask;
+    }
   }
 
   T F[DIM][DIM], Jri[DIM][DIM], detJr = T(1);
@@ -1258,6 +1271,13 @@ __global__ void __launch_bounds__(tile_threads<TILE>(), TILE == kTileWide ? 4 : 
       }
     }
     lds_barrier();
+    if constexpr (TILE) {
+      // every lane has gathered: the tile's destination values take the
+      // source values' place (out of registers until the final write)
+      if (t < TN)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) tile_v[t * DIM + c] = dv[c];
+    }
     if constexpr (!(PH & PH_SF)) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) out[c] = A[c * NQP + r];
@@ -1418,13 +1438,20 @@ __global__ void __launch_bounds__(tile_threads<TILE>(), TILE == kTileWide ? 4 : 
 
   if constexpr (TILE) {
     // sum the cells' results per tile node in LDS (fixed order), then one
-    // read-modify-write per node (distribute_local_to_global, :769-780)
-    lds_barrier();  // every lane is done with the sum-factorization buffers
-    T *res = lds;   // CPW * NQP * DIM values, within those buffers
-    if (on)
+    // write per node (distribute_local_to_global, :769-780).  In 3D the
+    // results are read where the integration (or, without it, the gather)
+    // left them after its last barrier: component c of point rr of slot s
+    // at lds[s * SLOTW + c * NQP + rr]; in 2D they are copied out first.
+    T *res = lds;
+    if constexpr (DIM == 2) {
+      lds_barrier();  // every lane is done with the sum-factorization buffers
+      if (on)
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) res[t * DIM + c] = out[c];
-    lds_barrier();
+        for (int c = 0; c < DIM; ++c) res[t * DIM + c] = out[c];
+      lds_barrier();
+    } else {
+      (void)out;
+    }
     if (tvalid) {
       const int first = P.tinfo[2 * (size_t)tile], ext = P.tinfo[2 * (size_t)tile + 1];
       (void)first;
@@ -1441,13 +1468,14 @@ __global__ void __launch_bounds__(tile_threads<TILE>(), TILE == kTileWide ? 4 : 
         for (int cy = ylo; cy <= yhi; ++cy)
           for (int cx = xlo; cx <= xhi; ++cx) {
             const int rr = (X - cx * PP) + N * ((Y - cy * PP) + N * (Z - cz * PP));
-            const int lane = (cx + TS::TX * (cy + TS::TY * cz)) * NQP + rr;
+            const int s = cx + TS::TX * (cy + TS::TY * cz);
 #pragma unroll
-            for (int c = 0; c < DIM; ++c) acc[c] += res[lane * DIM + c];
+            for (int c = 0; c < DIM; ++c)
+              acc[c] += DIM == 3 ? res[s * SLOTW + c * NQP + rr] : res[(s * NQP + rr) * DIM + c];
           }
       T nv[DIM];
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) nv[c] = dv[c] + acc[c];  // dv = 0 on first touch
+      for (int c = 0; c < DIM; ++c) nv[c] = tile_v[t * DIM + c] + acc[c];  // destination: 0 on first touch
       if (!CONSTR || !((tcode >> kConsShift) & ((1u << DIM) - 1u))) {
         store_node<DIM, T>(dst + tnode * DIM, nv);
       } else {
@@ -1478,8 +1506,9 @@ __global__ void __launch_bounds__(tile_threads<TILE>(), TILE == kTileWide ? 4 : 
   if (P.hang) {
     // constrained nodes' results go to their masters.  Several lanes of one
     // cell can share a master, so one lane per cell applies them in lane
-    // order, after the cell's own writes (the colouring covers the masters
-    // a cell writes, so no other cell of this colour touches them).
+    // order, after the cell's own writes; the colouring covers the masters
+    // a cell writes, so no other cell of this colour touches them and the
+    // read-modify-writes need no atomics.
     lds_barrier();  // every lane is done with the sum-factorization buffers
     T *hres = lds;  // [lane][DIM] results
     if (on)
@@ -1496,7 +1525,7 @@ __global__ void __launch_bounds__(tile_threads<TILE>(), TILE == kTileWide ? 4 : 
 #pragma unroll
           for (int c = 0; c < DIM; ++c) {
             if (CONSTR && ((mc >> (kConsShift + c)) & 1u)) continue;
-            atomicAdd(dst + m * DIM + c, P.hw[j] * hres[(slot * NQP + rr) * DIM + c]);
+            dst[m * DIM + c] += P.hw[j] * hres[(slot * NQP + rr) * DIM + c];
           }
         }
       }

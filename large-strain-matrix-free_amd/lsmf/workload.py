@@ -15,7 +15,7 @@ import math
 import numpy as np
 
 __all__ = ["SplitMix64", "lame_from_E_nu", "inclusion_material", "lognormal_material",
-           "smooth_displacement", "CONFIGS", "build_config_mesh"]
+           "smooth_displacement", "CONFIGS", "build_config_mesh", "half_refined_box"]
 
 _M64 = (1 << 64) - 1
 
@@ -129,3 +129,107 @@ def build_config_mesh(cfg_id, n=None):
     else:
         mu, lam = lognormal_material(mesh.n_cells, 1.0, cfg["nu"], cfg["sigma"], SEED + cfg_id)
     return mesh, mu, lam
+
+
+def half_refined_box(p, n):
+    """unit cube whose half x > 1/2 is refined once: n/2 x n x n cells of
+    size 1/n and n x 2n x 2n of size 1/(2n), Q_p with Gauss-Lobatto support
+    points.  The fine cells' nodes on x = 1/2 that are not nodes of the
+    coarse cells hang on the coarse faces: each carries the coarse face's
+    Lagrange weights (the hanging-node lines of deal.II's
+    make_hanging_node_constraints).  Returns (mesh, (nodes, ptr, masters,
+    weights)) in the format of NeoHookOperator's node_constraints."""
+    from types import SimpleNamespace
+    assert n % 2 == 0
+    g = _gll01(p)
+    ncx, nf = n // 2, 2 * n
+    Ncx, Nc = ncx * p + 1, n * p + 1
+    Nfx, Nf = n * p + 1, nf * p + 1
+    n_coarse = Ncx * Nc * Nc
+
+    def pos(i, cells, h):
+        c = np.minimum(i // p, cells - 1)
+        return (c + g[i - c * p]) * h
+
+    # coarse nodes
+    ix, iy, iz = np.meshgrid(np.arange(Ncx), np.arange(Nc), np.arange(Nc), indexing="ij")
+    cxyz = np.stack([pos(ix, ncx, 1.0 / n), pos(iy, n, 1.0 / n), pos(iz, n, 1.0 / n)], axis=-1)
+    cxyz = cxyz.transpose(2, 1, 0, 3).reshape(-1, 3)  # x fastest
+    # fine nodes: local id jx + Nfx (jy + Nf jz); x = 1/2 + ...
+    jx, jy, jz = np.meshgrid(np.arange(Nfx), np.arange(Nf), np.arange(Nf), indexing="ij")
+    fxyz = np.stack([0.5 + pos(jx, n, 0.5 / n), pos(jy, nf, 0.5 / n), pos(jz, nf, 0.5 / n)], axis=-1)
+    fxyz = fxyz.transpose(2, 1, 0, 3).reshape(-1, 3)
+    gid = np.full(Nfx * Nf * Nf, -1, np.int64)
+    interior = np.ones(Nfx * Nf * Nf, bool)
+    # interface nodes (jx = 0): coincide with a coarse node, or hang
+    hang_local, hang_masters, hang_weights = [], [], []
+
+    def coarse_coord(j):
+        """fine interface index -> (coarse cell, local coordinate in [0, 1])"""
+        fcell = min(j // p, nf - 1)
+        xi = ((fcell % 2) + g[j - fcell * p]) / 2.0
+        return fcell // 2, xi
+
+    def lag(k, x):
+        v = 1.0
+        for m in range(p + 1):
+            if m != k:
+                v *= (x - g[m]) / (g[k] - g[m])
+        return v
+
+    for kz in range(Nf):
+        cz, xz = coarse_coord(kz)
+        for ky in range(Nf):
+            cy, xy = coarse_coord(ky)
+            loc = Nfx * (ky + Nf * kz)
+            interior[loc] = False
+            wy = [lag(b, xy) for b in range(p + 1)]
+            wz = [lag(c, xz) for c in range(p + 1)]
+            ms, ws = [], []
+            for c in range(p + 1):
+                for b in range(p + 1):
+                    w = wy[b] * wz[c]
+                    if abs(w) > 1e-14:
+                        ms.append((Ncx - 1) + Ncx * ((cy * p + b) + Nc * (cz * p + c)))
+                        ws.append(w)
+            if len(ms) == 1 and abs(ws[0] - 1.0) < 1e-12:
+                gid[loc] = ms[0]  # a coarse node
+            else:
+                hang_local.append(loc)
+                hang_masters.append(ms)
+                hang_weights.append(ws)
+    # number the remaining fine nodes after the coarse ones
+    rest = np.where(gid < 0)[0]
+    gid[rest] = n_coarse + np.arange(len(rest))
+    coords = np.empty((n_coarse + len(rest), 3))
+    coords[:n_coarse] = cxyz
+    coords[n_coarse:] = fxyz[rest]
+    N = p + 1
+    a = np.arange(N)
+    loc3 = (a[None, None, :] + Ncx * (a[None, :, None] + Nc * a[:, None, None])).reshape(-1)
+    cc = np.stack(np.meshgrid(np.arange(ncx), np.arange(n), np.arange(n), indexing="ij"), -1).transpose(2, 1, 0, 3)
+    cc = cc.reshape(-1, 3)
+    cn_c = (cc[:, 0] * p + Ncx * (cc[:, 1] * p + Nc * cc[:, 2] * p))[:, None] + loc3[None, :]
+    loc3f = (a[None, None, :] + Nfx * (a[None, :, None] + Nf * a[:, None, None])).reshape(-1)
+    fc = np.stack(np.meshgrid(np.arange(n), np.arange(nf), np.arange(nf), indexing="ij"), -1).transpose(2, 1, 0, 3)
+    fc = fc.reshape(-1, 3)
+    cn_f = gid[(fc[:, 0] * p + Nfx * (fc[:, 1] * p + Nf * fc[:, 2] * p))[:, None] + loc3f[None, :]]
+    cell_nodes = np.ascontiguousarray(np.concatenate([cn_c, cn_f]).astype(np.int32))
+    nodes = gid[np.array(hang_local, np.int64)].astype(np.int32)
+    ptr = np.concatenate([[0], np.cumsum([len(m) for m in hang_masters])]).astype(np.int32)
+    masters = np.concatenate([np.array(m, np.int32) for m in hang_masters]).astype(np.int32)
+    weights = np.concatenate([np.array(w) for w in hang_weights])
+    mesh = SimpleNamespace(dim=3, degree=p, coords=coords, cell_nodes=cell_nodes, n_cells=len(cell_nodes),
+                           n_nodes=len(coords), n_dofs=3 * len(coords), cell_colour=None)
+    return mesh, (nodes, ptr, masters, weights)
+
+
+def _gll01(p):
+    """Gauss-Lobatto points on [0, 1] (FE_Q's support points)"""
+    if p == 1:
+        return np.array([0.0, 1.0])
+    from numpy.polynomial import legendre as L
+    c = np.zeros(p + 1)
+    c[-1] = 1
+    r = np.sort(L.legroots(L.legder(c)).real)
+    return np.concatenate([[0.0], 0.5 * (r + 1), [1.0]])

@@ -97,3 +97,44 @@ def test_hanging_operator_matches_oracle(dim, p, caching, device):
     op.vmult(pw, wt, mask="plain")
     assert rel(du.cpu().numpy(), -pw.cpu().numpy()) < 1e-12
 
+
+
+@pytest.mark.parametrize("p", [1, 2])
+@pytest.mark.parametrize("caching", ["tensor4", "tensor2"])
+def test_half_refined_box_matches_oracle_and_is_deterministic(p, caching, device):
+    """the hanging-node throughput mesh (lsmf.half_refined_box) at 2 coarse
+    cells per side against the oracle, FP64 to 1e-12; the masters' sums are
+    read-modify-writes in a fixed order (no atomics), so two products are
+    bit-identical"""
+    import torch
+    log(f"[hanging] half_refined_box p={p} caching={caching}")
+    mesh, nc = lsmf.half_refined_box(p, 2)
+    rng = np.random.default_rng(11)
+    mu, lam = 1.0 + rng.random(mesh.n_cells), 1.5 + rng.random(mesh.n_cells)
+    C = hanging_mesh.constraint_matrix(mesh, nc)
+    x = mesh.coords
+    u = C @ (0.02 * np.sin(np.pi * (x @ np.arange(1, 4)))[:, None] * np.ones(3)).reshape(-1)
+    cons = np.zeros(mesh.n_dofs, np.uint8)
+    left = np.where(np.abs(x[:, 0]) < 1e-12)[0]
+    for c in range(3):
+        cons[left * 3 + c] = 1
+    ref = oracle.OracleOperator(3, p, mesh.cell_nodes, mesh.coords, cons, caching=caching)
+    ref.set_node_constraints(*nc)
+    ref.set_material(mu, lam)
+    ref.set_linearization(u)
+    ref.cache()
+    op = lsmf.NeoHookOperator(mesh, precision="fp64", caching=caching, constrained=cons, use_mesh_colouring=False,
+                              node_constraints=nc)
+    op.set_material(mu, lam)
+    op.set_linearization(torch.tensor(u, device=device))
+    op.cache()
+    v = rng.standard_normal(mesh.n_dofs)
+    src = torch.tensor(v, device=device)
+    y1, y2 = torch.full_like(src, np.nan), torch.full_like(src, np.nan)
+    op.vmult(y1, src)
+    op.vmult(y2, src)
+    assert torch.equal(y1, y2)
+    assert rel(y1.cpu().numpy(), ref.vmult(v)) < 1e-12
+    pl = torch.full_like(src, np.nan)
+    op.vmult(pl, src, mask="plain")
+    assert rel(pl.cpu().numpy(), ref.vmult(v, plain=True)) < 1e-12

@@ -79,3 +79,47 @@ def test_weights_reproduce_continuous_fields(dim, p):
     zf = (x[:, 2] ** p + 2) if dim == 3 else 1.0
     f = np.stack([(1 + c + x[:, 0] ** p) * (0.5 - x[:, 1] ** p) * zf for c in range(dim)], 1).reshape(-1)
     assert np.allclose(C @ f, f, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_half_refined_box_constraints(p):
+    """lsmf.half_refined_box (the hanging-node throughput mesh): every node is
+    used, the hanging nodes are the fine interface nodes that are not coarse
+    nodes ((2np+1)^2 - (np+1)^2 of them), their weights reproduce fields of
+    degree p on the coarse faces, and the constrained operator on it equals
+    D C^T K C D on the free DoFs (x = 0 clamped)"""
+    n = 2
+    mesh, nc = lsmf.half_refined_box(p, n)
+    assert mesh.n_cells == (n // 2) * n * n + n * (2 * n) ** 2
+    used = np.zeros(mesh.n_nodes, bool)
+    used[mesh.cell_nodes.ravel()] = True
+    assert used.all()
+    assert len(nc[0]) == (2 * n * p + 1) ** 2 - (n * p + 1) ** 2
+    assert np.allclose(mesh.coords[nc[0], 0], 0.5)
+    C = hanging_mesh.constraint_matrix(mesh, nc)
+    x = mesh.coords
+    f = np.stack([(1 + c + x[:, 0] ** p) * (0.5 - x[:, 1] ** p) * (x[:, 2] ** p + 2) for c in range(3)], 1)
+    assert np.allclose(C @ f.reshape(-1), f.reshape(-1), rtol=1e-13, atol=1e-13)
+    cons = np.zeros(mesh.n_dofs, np.uint8)
+    left = np.where(np.abs(x[:, 0]) < 1e-12)[0]
+    for c in range(3):
+        cons[left * 3 + c] = 1
+    rng = np.random.default_rng(5)
+    mu, lam = 1.0 + rng.random(mesh.n_cells), 1.5 + rng.random(mesh.n_cells)
+    u = C @ (0.02 * np.sin(np.pi * (x @ np.arange(1, 4)))[:, None] * np.ones(3)).reshape(-1)
+    op = oracle.OracleOperator(3, p, mesh.cell_nodes, mesh.coords, cons, caching="tensor4")
+    op.set_node_constraints(*nc)
+    op.set_material(mu, lam)
+    op.set_linearization(u)
+    op.cache()
+    K, _ = assemble_numpy(mesh, mu, lam, u)
+    D = np.diag(1.0 - cons)
+    hang = np.zeros(mesh.n_dofs, bool)
+    for h in nc[0]:
+        hang[h * 3:(h + 1) * 3] = True
+    A = D @ C.T @ K @ C @ D
+    ident = (cons == 1) | hang
+    A[ident, :] = 0.0
+    A[np.ix_(ident, ident)] = np.diag(np.ones(ident.sum()))
+    v = rng.standard_normal(mesh.n_dofs)
+    assert np.allclose(op.vmult(v), A @ v, rtol=1e-12, atol=1e-12 * np.abs(A @ v).max())
