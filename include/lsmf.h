@@ -109,10 +109,11 @@ typedef struct {
    * over j in [nc_ptr[k], nc_ptr[k+1]).  Masters must be unconstrained by
    * this list (closed constraints); a master's Dirichlet components read as
    * zero.  Constrained nodes' rows act as identity, like Dirichlet rows.
-   * 0 / NULL: none.  With constraints the cells layout is used, the colouring
-   * covers the masters a cell writes, and compute_diagonal probes the
-   * operator with vmults (one per colour of a distance-2 node colouring and
-   * component) instead of the closed form. */
+   * 0 / NULL: none.  With constraints the cells layout is used; the
+   * constrained nodes' results reach their masters in one pass after the
+   * colours (in a fixed order, so products are reproducible), and
+   * compute_diagonal probes the operator with vmults (one per colour of a
+   * distance-2 node colouring and component) instead of the closed form. */
   int n_node_constraints;
   const int32_t *nc_nodes;   /* [n_node_constraints] */
   const int32_t *nc_ptr;     /* [n_node_constraints + 1] */

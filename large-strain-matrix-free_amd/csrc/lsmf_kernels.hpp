@@ -957,11 +957,17 @@ template <typename Number> struct VmultParams {
   int curved;       // support points off the Q1 maps: Eulerian Q_p metric (scalar variants)
   const uint32_t *__restrict__ tnodes;  // tiled layout: [tile][TN] node codes
   const int32_t *__restrict__ tinfo;    // tiled layout: [tile][2] first cell, extents
-  // node (hanging) constraints, cells layout only; hang == nullptr: none
-  const int32_t *__restrict__ hang;     // [sorted cell][NQP] constraint index or -1
+  // node (hanging) constraints, cells layout only; hang == nullptr: none.
+  // Each (cell, constrained node) pair is an instance: its cell reads the
+  // weighted sum of the masters and leaves its result in hbuf, from where
+  // k_hang_gather adds it to the masters once all colours are done.
+  const int32_t *__restrict__ hang;     // [sorted cell][NQP] instance or -1
+  const int32_t *__restrict__ hcons;    // [instance] constraint index
   const int32_t *__restrict__ hptr;     // [n + 1]
   const uint32_t *__restrict__ hmaster; // master node | its Dirichlet bits
   const Number *__restrict__ hw;        // weights
+  Number *__restrict__ hbuf;            // [vector][instance][DIM] results
+  size_t hstride;                       // values per vector in hbuf
   // adjoint: per-cell outputs in mesh order (perm: sorted -> mesh cell)
   double *__restrict__ dmu;
   double *__restrict__ dlambda;
@@ -1251,13 +1257,14 @@ __global__ void __launch_bounds__(tile_threads<TILE>(), (vmult_min_waves<VAR, TI
         for (int c = 0; c < DIM; ++c) A[c * NQP + r] = tile_v[ln * DIM + c];
     } else if (active) {
       const int hidx = P.hang ? P.hang[cell_s * NQP + r_s] : -1;
+      const int hk = hidx >= 0 ? P.hcons[hidx] : 0;
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
         T v;
         if (hidx >= 0) {
           // a constrained node reads the weighted sum of its masters
           v = T(0);
-          for (int j = P.hptr[hidx]; j < P.hptr[hidx + 1]; ++j) {
+          for (int j = P.hptr[hk]; j < P.hptr[hk + 1]; ++j) {
             const uint32_t mc = P.hmaster[j];
             if (CONSTR && ((mc >> (kConsShift + c)) & 1u)) continue;
             v = fma(P.hw[j], src[(size_t)(mc & kNodeMask) * DIM + c], v);
@@ -1503,34 +1510,41 @@ __global__ void __launch_bounds__(tile_threads<TILE>(), (vmult_min_waves<VAR, TI
       dst[node * DIM + c] = old[c] + out[c];
     }
   }
-  if (P.hang) {
-    // constrained nodes' results go to their masters.  Several lanes of one
-    // cell can share a master, so one lane per cell applies them in lane
-    // order, after the cell's own writes; the colouring covers the masters
-    // a cell writes, so no other cell of this colour touches them and the
-    // read-modify-writes need no atomics.
-    lds_barrier();  // every lane is done with the sum-factorization buffers
-    T *hres = lds;  // [lane][DIM] results
-    if (on)
+  // a constrained node's result waits in hbuf for k_hang_gather
+  // (distribute_local_to_global with constraints, :769-780)
+  if (hidx >= 0) {
+    T *h = P.hbuf + (size_t)blockIdx.y * P.hstride + (size_t)hidx * DIM;
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) hres[t * DIM + c] = out[c];
-    __syncthreads();  // LDS results and the cell's own global writes are visible
-    if (active && r == 0) {
-      for (int rr = 0; rr < NQP; ++rr) {
-        const int k = P.hang[cell_s * NQP + rr];
-        if (k < 0) continue;
-        for (int j = P.hptr[k]; j < P.hptr[k + 1]; ++j) {
-          const uint32_t mc = P.hmaster[j];
-          const size_t m = mc & kNodeMask;
-#pragma unroll
-          for (int c = 0; c < DIM; ++c) {
-            if (CONSTR && ((mc >> (kConsShift + c)) & 1u)) continue;
-            dst[m * DIM + c] += P.hw[j] * hres[(slot * NQP + rr) * DIM + c];
-          }
-        }
-      }
-    }
+    for (int c = 0; c < DIM; ++c) h[c] = out[c];
   }
+}
+
+// After all colours: every master adds the weighted results of the
+// constrained-node instances that name it, in instance order (fixed, so the
+// sums are reproducible), to its components that are not Dirichlet.  One
+// thread per master, grid y = vector.
+template <int DIM, typename T, bool CONSTR>
+__global__ void k_hang_gather(T *__restrict__ dst, size_t vstride, const T *__restrict__ hbuf, size_t hstride,
+                              const uint32_t *__restrict__ node, const int32_t *__restrict__ ptr,
+                              const int32_t *__restrict__ inst, const T *__restrict__ w, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const uint32_t code = node[q];
+  const size_t m = code & kNodeMask;
+  const T *hb = hbuf + (size_t)blockIdx.y * hstride;
+  T acc[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) acc[c] = T(0);
+  for (int e = ptr[q]; e < ptr[q + 1]; ++e) {
+    const T we = w[e];
+    const T *h = hb + (size_t)inst[e] * DIM;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) acc[c] = fma(we, h[c], acc[c]);
+  }
+  T *d = dst + (size_t)blockIdx.y * vstride + m * DIM;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c)
+    if (!CONSTR || !((code >> (kConsShift + c)) & 1u)) d[c] += acc[c];
 }
 
 template <typename T> __global__ void k_zero(T *__restrict__ p, size_t n) {

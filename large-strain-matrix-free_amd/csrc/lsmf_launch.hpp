@@ -54,6 +54,9 @@ template <int DIM, int N> struct Launch {
     P.tnodes = op->d_tile_nodes;
     P.tinfo = op->d_tile_info;
     P.hang = op->d_hang;
+    P.hcons = op->d_hcons;
+    P.hbuf = static_cast<T *>(op->d_hbuf);
+    P.hstride = (size_t)op->n_hang_inst * DIM;
     P.hptr = op->d_hptr;
     P.hmaster = op->d_hmaster;
     P.hw = static_cast<const T *>(op->d_hw);
@@ -93,6 +96,17 @@ template <int DIM, int N> struct Launch {
     return (int)(op->tiled ? op->tile_ptr.size() : op->colour_ptr.size()) - 1;
   }
 
+  // the masters' share of the constrained nodes' results, after the colours
+  // (nv vectors at dst + b * vstride)
+  template <typename T, bool CONSTR> static void hang_gather(const lsmf_op *op, T *dst, size_t vstride, unsigned nv,
+                                                              hipStream_t s) {
+    if (op->n_hang_masters == 0) return;
+    hipLaunchKernelGGL((k_hang_gather<DIM, T, CONSTR>), dim3((op->n_hang_masters + 255) / 256, nv), dim3(256), 0, s,
+                       dst, vstride, static_cast<const T *>(op->d_hbuf), (size_t)op->n_hang_inst * DIM,
+                       op->d_hg_node, op->d_hg_ptr, op->d_hg_inst, static_cast<const T *>(op->d_hg_w),
+                       op->n_hang_masters);
+  }
+
   // one launch per colour; the tiled layout launches one workgroup per tile
   template <typename T, int VAR, bool ADD, bool CONSTR, unsigned PH>
   static int run_colours(const lsmf_op *op, T *dst, const T *src, hipStream_t s) {
@@ -108,6 +122,7 @@ template <int DIM, int N> struct Launch {
       if (op->prof) op->prof->after(s);
       if (rc) return rc;
     }
+    if (op->n_node_cons > 0) hang_gather<T, CONSTR>(op, dst, 0, 1, s);
     return 0;
   }
 
@@ -146,7 +161,8 @@ template <int DIM, int N> struct Launch {
   template <typename T, int VAR> static int vmult_batched_t(const lsmf_op *op, T *dst, const T *src, int nvec,
                                                             hipStream_t s) {
     constexpr unsigned FULL = PH_SF | PH_QD;
-    constexpr int kMaxY = 65535;
+    // grid y holds the vectors; with node constraints, as many as d_hbuf holds
+    const int kMaxY = op->n_node_cons > 0 ? kHangBatch : 65535;
     const size_t n = op->n_dofs;
     const int nc = n_colours(op);
     for (int v0 = 0; v0 < nvec; v0 += kMaxY) {
@@ -168,6 +184,7 @@ template <int DIM, int N> struct Launch {
         const int rc = launch_vmult<T, VAR, false, true, FULL>(op, P, nb, nv, s);
         if (rc) return rc;
       }
+      if (op->n_node_cons > 0) hang_gather<T, true>(op, d, n, nv, s);
       if (op->n_cons > 0)
         hipLaunchKernelGGL(k_constrained_identity_batch<T>, dim3((op->n_cons + 255) / 256, nv), dim3(256), 0, s,
                            op->d_cons, op->n_cons, x, d, n);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -81,7 +82,8 @@ void free_op(lsmf_op *op) {
   void *ptrs[] = {op->d_idx,  op->d_coords,    op->d_tab,  op->d_tab64, op->d_mu64,     op->d_lambda64,
                   op->d_mu,   op->d_lambda,    op->d_cache, op->d_diag, op->d_inv_diag, op->d_cons,
                   op->d_bad,  op->d_perm,      op->d_tile_nodes, op->d_tile_info, op->d_orphan, op->d_hang, op->d_hptr,
-                  op->d_hmaster, op->d_hw, op->d_node_colour, op->d_fixed, op->d_probe};
+                  op->d_hmaster, op->d_hw, op->d_node_colour, op->d_fixed, op->d_probe, op->d_hcons,
+                  op->d_hg_node, op->d_hg_ptr, op->d_hg_inst, op->d_hg_w, op->d_hbuf};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (op->d_u64) (void)hipFree(op->d_u64);
@@ -533,8 +535,15 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
     }
   }
   op->n_node_cons = nnc;
-  // nodes cell c writes: its own nodes and the masters of its constrained ones
+  // nodes cell c writes in its colour: its own nodes (the masters of its
+  // constrained ones receive their share after the colours)
   auto write_nodes = [&](int c, std::vector<int> &w) {
+    w.clear();
+    for (int a = 0; a < nqp; ++a) w.push_back(mesh->cell_nodes[(size_t)c * nqp + a]);
+  };
+  // nodes whose values cell c couples: its own nodes and the masters of its
+  // constrained ones
+  auto coupled_nodes = [&](int c, std::vector<int> &w) {
     w.clear();
     for (int a = 0; a < nqp; ++a) {
       const int n = mesh->cell_nodes[(size_t)c * nqp + a];
@@ -675,11 +684,19 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   std::vector<uint32_t> hmaster;
   std::vector<double> hw;
   std::vector<uint8_t> fixed;
+  std::vector<int32_t> hcons, hg_ptr, hg_inst;
+  std::vector<uint32_t> hg_node;
+  std::vector<double> hg_w;
   if (nnc > 0) {
-    hang_tab.resize((size_t)nc * nqp);
+    // instances in sorted-cell order
+    hang_tab.assign((size_t)nc * nqp, -1);
     for (int pos = 0; pos < nc; ++pos)
-      for (int a = 0; a < nqp; ++a)
-        hang_tab[(size_t)pos * nqp + a] = hang_of[mesh->cell_nodes[(size_t)op->perm[pos] * nqp + a]];
+      for (int a = 0; a < nqp; ++a) {
+        const int k = hang_of[mesh->cell_nodes[(size_t)op->perm[pos] * nqp + a]];
+        if (k < 0) continue;
+        hang_tab[(size_t)pos * nqp + a] = (int32_t)hcons.size();
+        hcons.push_back(k);
+      }
     hptr.assign(mesh->nc_ptr, mesh->nc_ptr + nnc + 1);
     for (int j = 0; j < mesh->nc_ptr[nnc]; ++j) {
       const int m = mesh->nc_masters[j];
@@ -690,11 +707,28 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       hmaster.push_back(code);
       hw.push_back(mesh->nc_weights[j]);
     }
+    // per master: (instance, weight) in instance order
+    std::vector<std::array<int64_t, 3>> ent;  // master, instance, constraint entry
+    for (int i = 0; i < (int)hcons.size(); ++i)
+      for (int j = mesh->nc_ptr[hcons[i]]; j < mesh->nc_ptr[hcons[i] + 1]; ++j)
+        ent.push_back({mesh->nc_masters[j], i, j});
+    std::sort(ent.begin(), ent.end());
+    for (size_t e = 0; e < ent.size(); ++e) {
+      if (e == 0 || ent[e][0] != ent[e - 1][0]) {
+        hg_ptr.push_back((int32_t)e);
+        hg_node.push_back(hmaster[ent[e][2]]);
+      }
+      hg_inst.push_back((int32_t)ent[e][1]);
+      hg_w.push_back(hw[ent[e][2]]);
+    }
+    hg_ptr.push_back((int32_t)ent.size());
+    op->n_hang_inst = (int)hcons.size();
+    op->n_hang_masters = (int)hg_node.size();
     fixed.assign(op->n_dofs, 0);
     for (int32_t d : cons) fixed[d] = 1;
     std::vector<std::vector<int>> node_cells(mesh->n_nodes);
     for (int c = 0; c < nc; ++c) {
-      write_nodes(c, wbuf);
+      coupled_nodes(c, wbuf);
       for (int n : wbuf) node_cells[n].push_back(c);
     }
     ncolour.assign(mesh->n_nodes, -1);
@@ -702,7 +736,7 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
     int n_colours = 0;
     for (int n = 0; n < mesh->n_nodes; ++n) {
       for (int c : node_cells[n]) {
-        write_nodes(c, wbuf);
+        coupled_nodes(c, wbuf);
         for (int m : wbuf)
           if (ncolour[m] >= 0) {
             if ((int)mark.size() <= ncolour[m]) mark.resize(ncolour[m] + 1, -1);
@@ -760,6 +794,13 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
                    (rc = dalloc(&op->d_hptr, hptr.size(), "constraint offsets")) ||
                    (rc = dalloc(&op->d_hmaster, hmaster.size(), "constraint masters")) ||
                    (rc = dalloc((char **)&op->d_hw, hw.size() * es, "constraint weights")) ||
+                   (rc = dalloc(&op->d_hcons, hcons.size(), "constraint instances")) ||
+                   (rc = dalloc(&op->d_hg_node, hg_node.size(), "master nodes")) ||
+                   (rc = dalloc(&op->d_hg_ptr, hg_ptr.size(), "master offsets")) ||
+                   (rc = dalloc(&op->d_hg_inst, hg_inst.size(), "master instances")) ||
+                   (rc = dalloc((char **)&op->d_hg_w, hg_w.size() * es, "master weights")) ||
+                   (rc = dalloc((char **)&op->d_hbuf, (size_t)kHangBatch * hcons.size() * dim * es,
+                                "constrained-node results")) ||
                    (rc = dalloc(&op->d_node_colour, ncolour.size(), "node colours")) ||
                    (rc = dalloc(&op->d_fixed, fixed.size(), "fixed DoFs")) ||
                    (rc = dalloc((char **)&op->d_probe, op->n_dofs * es, "probe vector"))))) {
@@ -808,6 +849,22 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
       } else {
         std::vector<float> hf(hw.begin(), hw.end());
         e = hipMemcpy(op->d_hw, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice);
+      }
+    }
+    if (e == hipSuccess && !hcons.empty())
+      e = hipMemcpy(op->d_hcons, hcons.data(), hcons.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !hg_node.empty())
+      e = hipMemcpy(op->d_hg_node, hg_node.data(), hg_node.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(op->d_hg_ptr, hg_ptr.data(), hg_ptr.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !hg_inst.empty())
+      e = hipMemcpy(op->d_hg_inst, hg_inst.data(), hg_inst.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !hg_w.empty()) {
+      if (op->precision == LSMF_FP64) {
+        e = hipMemcpy(op->d_hg_w, hg_w.data(), hg_w.size() * sizeof(double), hipMemcpyHostToDevice);
+      } else {
+        std::vector<float> wf(hg_w.begin(), hg_w.end());
+        e = hipMemcpy(op->d_hg_w, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice);
       }
     }
     if (e == hipSuccess)

@@ -43,6 +43,9 @@ struct Profiler {
 };
 }  // namespace lsmf
 
+// vectors per k_hang_gather pass of a batched product (the size of d_hbuf)
+constexpr int kHangBatch = 8;
+
 struct lsmf_op {
   lsmf::Profiler *prof = nullptr;
   // shape
@@ -86,11 +89,19 @@ struct lsmf_op {
   uint32_t *d_tile_nodes = nullptr;  // [n_tiles][TN] node | constraint bits | first-touch bit, or kNoNode
   int32_t *d_tile_info = nullptr;    // [n_tiles][2] first sorted cell, extents ex | ey << 8 | ez << 16
   // node (hanging) constraints: cells layout, all-add scatter after a zero
-  // pass, diagonal by probing over a distance-2 node colouring
+  // pass, diagonal by probing over a distance-2 node colouring.  Instances
+  // (cell, constrained node) leave their results in d_hbuf; the masters
+  // gather them after the colours (d_hg_*: per master its instances and
+  // weights, in instance order), so masters do not enter the cell colouring.
   int n_node_cons = 0;
-  int32_t *d_hang = nullptr, *d_hptr = nullptr;
+  int32_t *d_hang = nullptr, *d_hptr = nullptr, *d_hcons = nullptr;
   uint32_t *d_hmaster = nullptr;
   void *d_hw = nullptr;
+  int n_hang_inst = 0, n_hang_masters = 0;
+  uint32_t *d_hg_node = nullptr;  // master node | its Dirichlet bits
+  int32_t *d_hg_ptr = nullptr, *d_hg_inst = nullptr;
+  void *d_hg_w = nullptr;
+  void *d_hbuf = nullptr;  // kHangBatch vectors of n_hang_inst * dim values
   int32_t *d_node_colour = nullptr;
   int n_node_colours = 0;
   uint8_t *d_fixed = nullptr;  // per DoF: Dirichlet or constrained node

@@ -6,8 +6,8 @@ oracle, through the C ABI: a coarse cell next to a once-refined region
     tensor4, tensor2 and scalar caches
   * the probing diagonal and the residual (distributed to the masters)
   * the residual VJP's displacement part equals minus the plain operator
-  * the cells layout is used (the mesh is not structured), and the
-    colouring keeps masters of one colour's cells apart
+  * the cells layout is used (the mesh is not structured); the masters
+    receive the constrained nodes' results after the colours
 """
 import numpy as np
 import pytest
