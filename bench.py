@@ -307,6 +307,16 @@ def hanging_record(caching, precision, n=64):
                                            node_constraints=nc)
     out["without_constraints"] = vmult_record(mesh, mu, lam, ud, cons, caching, precision, 10, 2, layout="cells")
     out["overhead"] = round(out["with_constraints"]["ms"] / out["without_constraints"]["ms"] - 1.0, 4)
+    # compute_diagonal (closed form; cells with constrained nodes per coupled node)
+    for key, ncs in (("diagonal_ms_with_constraints", nc), ("diagonal_ms_without_constraints", None)):
+        op = lsmf.NeoHookOperator(mesh, precision=precision, caching=caching, constrained=cons, layout="cells",
+                                  use_mesh_colouring=False, node_constraints=ncs)
+        op.set_material(mu, lam)
+        op.set_linearization(ud)
+        op.cache()
+        out[key] = round(time_ms(op.compute_diagonal, 5, torch), 4)
+        del op
+        torch.cuda.empty_cache()
     del ud
     torch.cuda.empty_cache()
     return out

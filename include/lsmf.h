@@ -112,8 +112,9 @@ typedef struct {
    * 0 / NULL: none.  With constraints the cells layout is used; the
    * constrained nodes' results reach their masters in one pass after the
    * colours (in a fixed order, so products are reproducible), and
-   * compute_diagonal probes the operator with vmults (one per colour of a
-   * distance-2 node colouring and component) instead of the closed form. */
+   * compute_diagonal evaluates, on each cell with constrained nodes, the
+   * closed form for the constrained basis function of every node the cell
+   * couples (its own unconstrained nodes and the masters). */
   int n_node_constraints;
   const int32_t *nc_nodes;   /* [n_node_constraints] */
   const int32_t *nc_ptr;     /* [n_node_constraints + 1] */

@@ -968,6 +968,17 @@ template <typename Number> struct VmultParams {
   const Number *__restrict__ hw;        // weights
   Number *__restrict__ hbuf;            // [vector][instance][DIM] results
   size_t hstride;                       // values per vector in hbuf
+  // closed-form diagonal of cells with constrained nodes: per sorted cell
+  // its index among them or -1; per such cell its entries (one per node the
+  // cell couples: own unconstrained nodes and masters), per entry the local
+  // nodes and weights whose basis functions sum to the node's constrained
+  // basis function on the cell; results per entry in dbuf
+  const int32_t *__restrict__ dcell;
+  const int32_t *__restrict__ deptr;
+  const int32_t *__restrict__ dtptr;
+  const int32_t *__restrict__ dta;
+  const Number *__restrict__ dtw;
+  Number *__restrict__ dbuf;
   // adjoint: per-cell outputs in mesh order (perm: sorted -> mesh cell)
   double *__restrict__ dmu;
   double *__restrict__ dlambda;
@@ -1522,7 +1533,8 @@ __global__ void __launch_bounds__(tile_threads<TILE>(), (vmult_min_waves<VAR, TI
 // After all colours: every master adds the weighted results of the
 // constrained-node instances that name it, in instance order (fixed, so the
 // sums are reproducible), to its components that are not Dirichlet.  One
-// thread per master, grid y = vector.
+// thread per master, grid y = vector.  w == nullptr: unit weights (the
+// diagonal's per-entry results).
 template <int DIM, typename T, bool CONSTR>
 __global__ void k_hang_gather(T *__restrict__ dst, size_t vstride, const T *__restrict__ hbuf, size_t hstride,
                               const uint32_t *__restrict__ node, const int32_t *__restrict__ ptr,
@@ -1536,7 +1548,7 @@ __global__ void k_hang_gather(T *__restrict__ dst, size_t vstride, const T *__re
 #pragma unroll
   for (int c = 0; c < DIM; ++c) acc[c] = T(0);
   for (int e = ptr[q]; e < ptr[q + 1]; ++e) {
-    const T we = w[e];
+    const T we = w ? w[e] : T(1);
     const T *h = hb + (size_t)inst[e] * DIM;
 #pragma unroll
     for (int c = 0; c < DIM; ++c) acc[c] = fma(we, h[c], acc[c]);
@@ -1552,22 +1564,6 @@ template <typename T> __global__ void k_zero(T *__restrict__ p, size_t n) {
   if (i < n) p[i] = T(0);
 }
 
-// diagonal by probing (meshes with node constraints): x = indicator of the
-// free DoFs (node colour k, component c), and diag = (A x) on them
-template <typename T>
-__global__ void k_probe_set(T *__restrict__ x, const int32_t *__restrict__ node_colour,
-                            const uint8_t *__restrict__ fixed, int k, int c, int dim, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] = (node_colour[i / dim] == k && (int)(i % dim) == c && !fixed[i]) ? T(1) : T(0);
-}
-template <typename T>
-__global__ void k_probe_take(T *__restrict__ diag, const T *__restrict__ y, const int32_t *__restrict__ node_colour,
-                             const uint8_t *__restrict__ fixed, int k, int c, int dim, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && node_colour[i / dim] == k && (int)(i % dim) == c && !fixed[i]) diag[i] = y[i];
-}
-
-// constrained rows act as identity (include/mf_nh_operator.h:703-705)
 template <typename T>
 __global__ void k_constrained_identity(const int32_t *__restrict__ list, int n, const T *__restrict__ src,
                                        T *__restrict__ dst, int add) {
@@ -1829,21 +1825,21 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
   lds_barrier();
   if (!TILE && !active) return;  // the tiled layout's lanes all reach the sum below
   const T *S = tab, *D = tab + N * N;
-  const int ai[3] = {i0, i1, i2};
-  T acc[DIM];
-#pragma unroll
-  for (int c = 0; c < DIM; ++c) acc[c] = T(0);
-  for (int q = 0; q < NQP; ++q) {
-    const int q0 = q % N, q1 = (q / N) % N, q2 = (DIM == 3) ? q / (N * N) : 0;
-    const int qi[3] = {q0, q1, q2};
-    T gph[DIM];
+  // reference gradient of the basis function of local node a at point q
+  auto grad_ref = [&](int a, int q, T (&g)[DIM]) {
+    const int ai[3] = {a % N, (a / N) % N, (DIM == 3) ? a / (N * N) : 0};
+    const int qi[3] = {q % N, (q / N) % N, (DIM == 3) ? q / (N * N) : 0};
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
       T v = T(1);
 #pragma unroll
       for (int e = 0; e < DIM; ++e) v *= (e == d ? D : S)[qi[e] * N + ai[e]];
-      gph[d] = v;
+      g[d] = v;
     }
+  };
+  // point q's share of the diagonal entries (one per component) of the
+  // basis function with reference gradient gph
+  auto point = [&](int q, const T (&gph)[DIM], T (&acc)[DIM]) {
     T gam[DIM];
 #pragma unroll
     for (int j = 0; j < DIM; ++j) {
@@ -1865,7 +1861,7 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
           for (int f = 0; f < DIM; ++f) s = fma(gam[e] * gam[f], Am[sym_index<D2>(c * DIM + e, c * DIM + f) * NQP + q], s);
         acc[c] += s;
       }
-      continue;
+      return;
     }
     T geo = T(0);
 #pragma unroll
@@ -1897,6 +1893,43 @@ __global__ void __launch_bounds__(kWG) k_diagonal(const VmultParams<T> P) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) acc[c] += geo + T(0.5) * c1 * (g2 + gam[c] * gam[c]) + c2 * gam[c] * gam[c];
     }
+  };
+  if constexpr (!TILE) {
+    const int hc = P.dcell ? P.dcell[cell] : -1;
+    if (hc >= 0) {
+      // a cell with constrained nodes: per entry (a node the cell couples)
+      // the basis function is the weighted sum of local ones; results wait
+      // in dbuf for the gather that follows the colours
+      for (int e = P.deptr[hc] + r; e < P.deptr[hc + 1]; e += NQP) {
+        T acc[DIM];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) acc[c] = T(0);
+        for (int q = 0; q < NQP; ++q) {
+          T gph[DIM];
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) gph[d] = T(0);
+          for (int k = P.dtptr[e]; k < P.dtptr[e + 1]; ++k) {
+            T g[DIM];
+            grad_ref(P.dta[k], q, g);
+            const T w = P.dtw[k];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) gph[d] = fma(w, g[d], gph[d]);
+          }
+          point(q, gph, acc);
+        }
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) P.dbuf[(size_t)e * DIM + c] = acc[c];
+      }
+      return;
+    }
+  }
+  T acc[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) acc[c] = T(0);
+  for (int q = 0; q < NQP; ++q) {
+    T gph[DIM];
+    grad_ref(r, q, gph);
+    point(q, gph, acc);
   }
   if constexpr (TILE) {
     lds_barrier();  // every lane is done with the per-point fields

@@ -58,6 +58,12 @@ template <int DIM, int N> struct Launch {
     P.hbuf = static_cast<T *>(op->d_hbuf);
     P.hstride = (size_t)op->n_hang_inst * DIM;
     P.hptr = op->d_hptr;
+    P.dcell = op->d_dcell;
+    P.deptr = op->d_deptr;
+    P.dtptr = op->d_dtptr;
+    P.dta = op->d_dta;
+    P.dtw = static_cast<const T *>(op->d_dtw);
+    P.dbuf = static_cast<T *>(op->d_dbuf);
     P.hmaster = op->d_hmaster;
     P.hw = static_cast<const T *>(op->d_hw);
     P.dmu = op->adj_dmu;
@@ -270,7 +276,12 @@ template <int DIM, int N> struct Launch {
   }
 
   template <typename T, int VAR> static int diagonal_t(const lsmf_op *op, hipStream_t s) {
-    if (op->n_node_cons > 0) return diagonal_probe_t<T, VAR>(op, s);
+    // with node constraints every cell adds (no first-touch bits): start
+    // from zero; cells with constrained nodes leave per-entry results that
+    // the gather below adds per node
+    if (op->n_node_cons > 0)
+      hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((op->n_dofs + 255) / 256)), dim3(256), 0, s,
+                         static_cast<T *>(op->d_diag), op->n_dofs);
     const int nc = n_colours(op);
     for (int k = 0; k < nc; ++k) {
       VmultParams<T> P = params<T>(op, k);
@@ -292,6 +303,10 @@ template <int DIM, int N> struct Launch {
       }
     }
     T *diag = static_cast<T *>(op->d_diag);
+    if (op->n_diag_nodes > 0)
+      hipLaunchKernelGGL((k_hang_gather<DIM, T, true>), dim3((op->n_diag_nodes + 255) / 256), dim3(256), 0, s, diag,
+                         (size_t)0, static_cast<const T *>(op->d_dbuf), (size_t)0, op->d_dg_node, op->d_dg_ptr,
+                         op->d_dg_ent, static_cast<const T *>(nullptr), op->n_diag_nodes);
     zero_orphans(op, diag, s);
     if (op->n_cons > 0)
       hipLaunchKernelGGL(k_set_one<T>, dim3((op->n_cons + 255) / 256), dim3(256), 0, s, op->d_cons, op->n_cons, diag);
@@ -302,31 +317,6 @@ template <int DIM, int N> struct Launch {
     return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
   }
 
-  // node constraints: diag(A) by probing, one vmult per node colour and
-  // component (nodes a common cell writes differ in colour, so each probe
-  // reads its DoFs' diagonal entries without interference)
-  template <typename T, int VAR> static int diagonal_probe_t(const lsmf_op *op, hipStream_t s) {
-    T *diag = static_cast<T *>(op->d_diag);
-    T *x = static_cast<T *>(op->d_inv_diag);  // scratch until inverted below
-    T *y = static_cast<T *>(op->d_probe);
-    const size_t n = op->n_dofs;
-    const dim3 g((unsigned)((n + 255) / 256)), b(256);
-    constexpr unsigned FULL = PH_SF | PH_QD;
-    hipLaunchKernelGGL(k_zero<T>, g, b, 0, s, diag, n);
-    for (int k = 0; k < op->n_node_colours; ++k)
-      for (int c = 0; c < DIM; ++c) {
-        hipLaunchKernelGGL(k_probe_set<T>, g, b, 0, s, x, op->d_node_colour, op->d_fixed, k, c, DIM, n);
-        zero_if_constrained(op, y, false, s);
-        const int rc = run_colours<T, VAR, false, true, FULL>(op, y, x, s);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_probe_take<T>, g, b, 0, s, diag, y, op->d_node_colour, op->d_fixed, k, c, DIM, n);
-      }
-    if (op->n_cons > 0)
-      hipLaunchKernelGGL(k_set_one<T>, dim3((op->n_cons + 255) / 256), dim3(256), 0, s, op->d_cons, op->n_cons, diag);
-    hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(64), 0, s, op->d_bad, (size_t)1);
-    hipLaunchKernelGGL(k_invert<T>, g, b, 0, s, diag, static_cast<T *>(op->d_inv_diag), n, op->d_bad);
-    return hipGetLastError() == hipSuccess ? 0 : LSMF_ERR_HIP;
-  }
 
   static int diagonal(const lsmf_op *op, hipStream_t s) {
     const bool f64 = op->precision == LSMF_FP64;

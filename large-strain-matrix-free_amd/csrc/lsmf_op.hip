@@ -82,8 +82,9 @@ void free_op(lsmf_op *op) {
   void *ptrs[] = {op->d_idx,  op->d_coords,    op->d_tab,  op->d_tab64, op->d_mu64,     op->d_lambda64,
                   op->d_mu,   op->d_lambda,    op->d_cache, op->d_diag, op->d_inv_diag, op->d_cons,
                   op->d_bad,  op->d_perm,      op->d_tile_nodes, op->d_tile_info, op->d_orphan, op->d_hang, op->d_hptr,
-                  op->d_hmaster, op->d_hw, op->d_node_colour, op->d_fixed, op->d_probe, op->d_hcons,
-                  op->d_hg_node, op->d_hg_ptr, op->d_hg_inst, op->d_hg_w, op->d_hbuf};
+                  op->d_hmaster, op->d_hw, op->d_hcons, op->d_hg_node, op->d_hg_ptr, op->d_hg_inst, op->d_hg_w,
+                  op->d_hbuf, op->d_dcell, op->d_deptr, op->d_dtptr, op->d_dta, op->d_dtw, op->d_dbuf,
+                  op->d_dg_node, op->d_dg_ptr, op->d_dg_ent};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (op->d_u64) (void)hipFree(op->d_u64);
@@ -541,17 +542,6 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
     w.clear();
     for (int a = 0; a < nqp; ++a) w.push_back(mesh->cell_nodes[(size_t)c * nqp + a]);
   };
-  // nodes whose values cell c couples: its own nodes and the masters of its
-  // constrained ones
-  auto coupled_nodes = [&](int c, std::vector<int> &w) {
-    w.clear();
-    for (int a = 0; a < nqp; ++a) {
-      const int n = mesh->cell_nodes[(size_t)c * nqp + a];
-      w.push_back(n);
-      if (nnc > 0 && hang_of[n] >= 0)
-        for (int j = mesh->nc_ptr[hang_of[n]]; j < mesh->nc_ptr[hang_of[n] + 1]; ++j) w.push_back(mesh->nc_masters[j]);
-    }
-  };
   std::vector<int> wbuf;
 
   // --- colouring (given, checked below, or greedy first fit)
@@ -678,12 +668,14 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
   }
   op->n_orphan = (int)orphan.size();
 
-  // --- node-constraint tables and the distance-2 node colouring that the
-  // probing diagonal uses (nodes written by a common cell differ in colour)
-  std::vector<int32_t> hang_tab, hptr, ncolour;
+  // --- node-constraint tables: instances and their masters (products), and
+  // the per-entry tables of the closed-form diagonal
+  std::vector<int32_t> hang_tab, hptr;
   std::vector<uint32_t> hmaster;
   std::vector<double> hw;
-  std::vector<uint8_t> fixed;
+  std::vector<int32_t> dcell, deptr, dtptr, dta, dg_ptr, dg_ent, dent_node;
+  std::vector<double> dtw;
+  std::vector<uint32_t> dg_node;
   std::vector<int32_t> hcons, hg_ptr, hg_inst;
   std::vector<uint32_t> hg_node;
   std::vector<double> hg_w;
@@ -724,31 +716,66 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
     hg_ptr.push_back((int32_t)ent.size());
     op->n_hang_inst = (int)hcons.size();
     op->n_hang_masters = (int)hg_node.size();
-    fixed.assign(op->n_dofs, 0);
-    for (int32_t d : cons) fixed[d] = 1;
-    std::vector<std::vector<int>> node_cells(mesh->n_nodes);
-    for (int c = 0; c < nc; ++c) {
-      coupled_nodes(c, wbuf);
-      for (int n : wbuf) node_cells[n].push_back(c);
-    }
-    ncolour.assign(mesh->n_nodes, -1);
-    std::vector<int> mark;
-    int n_colours = 0;
-    for (int n = 0; n < mesh->n_nodes; ++n) {
-      for (int c : node_cells[n]) {
-        coupled_nodes(c, wbuf);
-        for (int m : wbuf)
-          if (ncolour[m] >= 0) {
-            if ((int)mark.size() <= ncolour[m]) mark.resize(ncolour[m] + 1, -1);
-            mark[ncolour[m]] = n;
-          }
+    // closed-form diagonal: per cell with constrained nodes, one entry per
+    // node it couples (first appearance over the local nodes), each the
+    // weighted sum of local basis functions that is that node's constrained
+    // basis function on the cell; then per node its entries
+    dcell.assign(nc, -1);
+    deptr.push_back(0);
+    dtptr.push_back(0);
+    std::vector<int> eg;                               // entry -> node
+    std::vector<std::vector<std::pair<int, double>>> et;  // entry -> (local node, weight)
+    for (int pos = 0; pos < nc; ++pos) {
+      const int c = op->perm[pos];
+      bool any = false;
+      for (int a = 0; a < nqp && !any; ++a) any = hang_of[mesh->cell_nodes[(size_t)c * nqp + a]] >= 0;
+      if (!any) continue;
+      dcell[pos] = (int32_t)(deptr.size() - 1);
+      eg.clear();
+      et.clear();
+      auto add = [&](int g, int a, double w) {
+        size_t k = 0;
+        while (k < eg.size() && eg[k] != g) ++k;
+        if (k == eg.size()) {
+          eg.push_back(g);
+          et.emplace_back();
+        }
+        et[k].push_back({a, w});
+      };
+      for (int a = 0; a < nqp; ++a) {
+        const int n = mesh->cell_nodes[(size_t)c * nqp + a];
+        const int k = hang_of[n];
+        if (k < 0) add(n, a, 1.0);
+        else
+          for (int j = mesh->nc_ptr[k]; j < mesh->nc_ptr[k + 1]; ++j) add(mesh->nc_masters[j], a, mesh->nc_weights[j]);
       }
-      int col = 0;
-      while (col < (int)mark.size() && mark[col] == n) ++col;
-      ncolour[n] = col;
-      n_colours = std::max(n_colours, col + 1);
+      for (size_t k = 0; k < eg.size(); ++k) {
+        dent_node.push_back(eg[k]);
+        for (const auto &tw : et[k]) {
+          dta.push_back(tw.first);
+          dtw.push_back(tw.second);
+        }
+        dtptr.push_back((int32_t)dta.size());
+      }
+      deptr.push_back((int32_t)dent_node.size());
     }
-    op->n_node_colours = n_colours;
+    std::vector<std::pair<int, int>> ne;  // (node, entry)
+    for (int e = 0; e < (int)dent_node.size(); ++e) ne.push_back({dent_node[e], e});
+    std::sort(ne.begin(), ne.end());
+    for (size_t k = 0; k < ne.size(); ++k) {
+      if (k == 0 || ne[k].first != ne[k - 1].first) {
+        const int m = ne[k].first;
+        uint32_t code = (uint32_t)m;
+        if (mesh->constrained)
+          for (int c = 0; c < dim; ++c)
+            if (mesh->constrained[(size_t)m * dim + c]) code |= 1u << (kConsShift + c);
+        dg_ptr.push_back((int32_t)k);
+        dg_node.push_back(code);
+      }
+      dg_ent.push_back(ne[k].second);
+    }
+    dg_ptr.push_back((int32_t)ne.size());
+    op->n_diag_nodes = (int)dg_node.size();
   }
 
   // --- 1D tables: S, D, St, Dt, qp, qw, and the collocation derivative on
@@ -801,9 +828,15 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
                    (rc = dalloc((char **)&op->d_hg_w, hg_w.size() * es, "master weights")) ||
                    (rc = dalloc((char **)&op->d_hbuf, (size_t)kHangBatch * hcons.size() * dim * es,
                                 "constrained-node results")) ||
-                   (rc = dalloc(&op->d_node_colour, ncolour.size(), "node colours")) ||
-                   (rc = dalloc(&op->d_fixed, fixed.size(), "fixed DoFs")) ||
-                   (rc = dalloc((char **)&op->d_probe, op->n_dofs * es, "probe vector"))))) {
+                   (rc = dalloc(&op->d_dcell, dcell.size(), "diagonal cells")) ||
+                   (rc = dalloc(&op->d_deptr, deptr.size(), "diagonal entries")) ||
+                   (rc = dalloc(&op->d_dtptr, dtptr.size(), "diagonal terms")) ||
+                   (rc = dalloc(&op->d_dta, dta.size(), "diagonal local nodes")) ||
+                   (rc = dalloc((char **)&op->d_dtw, dtw.size() * es, "diagonal weights")) ||
+                   (rc = dalloc((char **)&op->d_dbuf, dent_node.size() * dim * es, "diagonal entry results")) ||
+                   (rc = dalloc(&op->d_dg_node, dg_node.size(), "diagonal nodes")) ||
+                   (rc = dalloc(&op->d_dg_ptr, dg_ptr.size(), "diagonal node offsets")) ||
+                   (rc = dalloc(&op->d_dg_ent, dg_ent.size(), "diagonal node entries"))))) {
     free_op(op);
     delete op;
     return rc;
@@ -867,9 +900,22 @@ lsmf_status lsmf_op_create(const lsmf_mesh_desc *mesh, const lsmf_op_options *op
         e = hipMemcpy(op->d_hg_w, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice);
       }
     }
-    if (e == hipSuccess)
-      e = hipMemcpy(op->d_node_colour, ncolour.data(), ncolour.size() * sizeof(int32_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(op->d_fixed, fixed.data(), fixed.size(), hipMemcpyHostToDevice);
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+      if (e == hipSuccess && bytes > 0) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    up(op->d_dcell, dcell.data(), dcell.size() * sizeof(int32_t));
+    up(op->d_deptr, deptr.data(), deptr.size() * sizeof(int32_t));
+    up(op->d_dtptr, dtptr.data(), dtptr.size() * sizeof(int32_t));
+    up(op->d_dta, dta.data(), dta.size() * sizeof(int32_t));
+    up(op->d_dg_node, dg_node.data(), dg_node.size() * sizeof(uint32_t));
+    up(op->d_dg_ptr, dg_ptr.data(), dg_ptr.size() * sizeof(int32_t));
+    up(op->d_dg_ent, dg_ent.data(), dg_ent.size() * sizeof(int32_t));
+    if (op->precision == LSMF_FP64) {
+      up(op->d_dtw, dtw.data(), dtw.size() * sizeof(double));
+    } else {
+      std::vector<float> wf(dtw.begin(), dtw.end());
+      up(op->d_dtw, wf.data(), wf.size() * sizeof(float));
+    }
   }
   op->bytes_idx += tnodes.size() * sizeof(uint32_t) + tinfo.size() * sizeof(int32_t);
   if (e != hipSuccess) {

@@ -89,7 +89,7 @@ struct lsmf_op {
   uint32_t *d_tile_nodes = nullptr;  // [n_tiles][TN] node | constraint bits | first-touch bit, or kNoNode
   int32_t *d_tile_info = nullptr;    // [n_tiles][2] first sorted cell, extents ex | ey << 8 | ez << 16
   // node (hanging) constraints: cells layout, all-add scatter after a zero
-  // pass, diagonal by probing over a distance-2 node colouring.  Instances
+  // pass.  Instances
   // (cell, constrained node) leave their results in d_hbuf; the masters
   // gather them after the colours (d_hg_*: per master its instances and
   // weights, in instance order), so masters do not enter the cell colouring.
@@ -102,10 +102,14 @@ struct lsmf_op {
   int32_t *d_hg_ptr = nullptr, *d_hg_inst = nullptr;
   void *d_hg_w = nullptr;
   void *d_hbuf = nullptr;  // kHangBatch vectors of n_hang_inst * dim values
-  int32_t *d_node_colour = nullptr;
-  int n_node_colours = 0;
-  uint8_t *d_fixed = nullptr;  // per DoF: Dirichlet or constrained node
-  void *d_probe = nullptr;     // probing diagonal: A x
+  // closed-form diagonal with node constraints (VmultParams::dcell ...):
+  // cells with constrained nodes evaluate one entry per node they couple;
+  // d_dg_*: per node its entries, in entry order, for the gather
+  int32_t *d_dcell = nullptr, *d_deptr = nullptr, *d_dtptr = nullptr, *d_dta = nullptr;
+  void *d_dtw = nullptr, *d_dbuf = nullptr;
+  int n_diag_nodes = 0;
+  uint32_t *d_dg_node = nullptr;
+  int32_t *d_dg_ptr = nullptr, *d_dg_ent = nullptr;
   bool material_set = false, linearization_set = false, cached = false, diag_ok = false;
   size_t bytes_idx = 0, bytes_cache = 0;
 };
