@@ -577,12 +577,10 @@ template <typename T> class GMG : public GMGBase {
     const size_t n = v.g->n_dofs;
     const T *invd = v.op->get_matrix_diagonal_inverse();
     if (x_zero) {
-      hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(256), 0, s_, x, v.d.p, b, (const T *)nullptr, invd,
-                         1.0 / v.theta, 1, n);
+      launch_cheb_first<T>(s_, x, v.d.p, b, (const T *)nullptr, invd, 1.0 / v.theta, 1, n);
     } else {
       v.op->vmult(v.t.p, x);
-      hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(256), 0, s_, x, v.d.p, b, (const T *)v.t.p, invd,
-                         1.0 / v.theta, 0, n);
+      launch_cheb_first<T>(s_, x, v.d.p, b, (const T *)v.t.p, invd, 1.0 / v.theta, 0, n);
     }
     if (v.degree < 2 || std::fabs(v.delta) < 1e-40) return;
     double rhok = v.delta / v.theta;
@@ -592,8 +590,7 @@ template <typename T> class GMG : public GMGBase {
       const double rhokp = 1.0 / (2.0 * sigma - rhok);
       const double f1 = rhokp * rhok, f2 = 2.0 * rhokp / v.delta;
       rhok = rhokp;
-      hipLaunchKernelGGL(k_cheb_update<T>, dim3(nblk(n)), dim3(256), 0, s_, x, v.d.p, b, (const T *)v.t.p, invd, f1, f2,
-                         n);
+      launch_cheb_update<T>(s_, x, v.d.p, b, (const T *)v.t.p, invd, f1, f2, n);
     }
   }
 
@@ -613,7 +610,7 @@ template <typename T> class GMG : public GMGBase {
     }
     chebyshev(v, v.sol.p, v.rhs.p, true);  // pre-smoothing from zero
     v.op->vmult(v.t.p, v.sol.p);
-    hipLaunchKernelGGL(k_residual<T>, dim3(nblk(n)), dim3(256), 0, s_, v.t.p, (const T *)v.rhs.p, n);
+    launch_residual<T>(s_, v.t.p, (const T *)v.rhs.p, n);
     Level<T> &c = lv_[l - 1];
     restrict_(l, c.rhs.p, v.t.p);
     vcycle(l - 1);
@@ -649,7 +646,7 @@ template <typename T> class GMG : public GMGBase {
       const double rz_new = red_.dot(r, z, n);
       const double beta = rz_new / rz;
       rz = rz_new;
-      hipLaunchKernelGGL(k_xpby<T>, dim3(nblk(n)), dim3(256), 0, s_, p, (const T *)z, beta, n);
+      launch_xpby<T>(s_, p, (const T *)z, beta, n);
     }
     throw HipError(LSMF_ERR_NOT_CONVERGED, "coarse CG: no convergence");
   }
@@ -663,7 +660,7 @@ template <typename T> class GMG : public GMGBase {
     hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, s_, x, T(0), n);
     hipLaunchKernelGGL(k_pseudo_random<T>, dim3(nblk(n)), dim3(256), 0, s_, r, n, (uint64_t)20261202ull);
     zero_constrained((int)(&v - lv_.data()), r);
-    hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(n)), dim3(256), 0, s_, z, (const T *)r, invd, n);
+    launch_jacobi<T>(s_, z, (const T *)r, invd, n);
     hip_check(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, s_), "eig copy");
     double rz = red_.dot(r, z, n);
     const double r0 = std::sqrt(red_.dot(r, r, n));
@@ -676,12 +673,12 @@ template <typename T> class GMG : public GMGBase {
       alpha.push_back(a);
       const double rr = red_.cg_update(x, r, p, Ap, a, n);
       if (std::sqrt(rr) <= 1e-10 * r0) break;
-      hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(n)), dim3(256), 0, s_, z, (const T *)r, invd, n);
+      launch_jacobi<T>(s_, z, (const T *)r, invd, n);
       const double rz_new = red_.dot(r, z, n);
       const double b = rz_new / rz;
       beta.push_back(b);
       rz = rz_new;
-      hipLaunchKernelGGL(k_xpby<T>, dim3(nblk(n)), dim3(256), 0, s_, p, (const T *)z, b, n);
+      launch_xpby<T>(s_, p, (const T *)z, b, n);
     }
     const size_t m = alpha.size();
     if (m == 0) throw HipError(LSMF_ERR_HIP, "eigenvalue estimate: no CG step");
@@ -916,7 +913,7 @@ double condition_number(lsmf_solver *S, const double *b, int its, double tol) {
     const double bcoef = rr_new / rr;
     beta.push_back(bcoef);
     rr = rr_new;
-    hipLaunchKernelGGL(k_xpby<double>, dim3(nblk(n)), dim3(256), 0, s, p, (const double *)r, bcoef, n);
+    launch_xpby<double>(s, p, (const double *)r, bcoef, n);
   }
   const size_t m = alpha.size();
   if (m < 2) return m == 1 ? 1.0 : 0.0;
@@ -973,7 +970,7 @@ int pcg(lsmf_solver *S, double *x, const double *b, double tol, int max_it, doub
     const double rz_new = red.dot(S->r.p, S->z.p, n);
     const double beta = rz_new / rz;
     rz = rz_new;
-    hipLaunchKernelGGL(k_xpby<double>, dim3(nblk(n)), dim3(256), 0, s, S->p.p, (const double *)S->z.p, beta, n);
+    launch_xpby<double>(s, S->p.p, (const double *)S->z.p, beta, n);
   }
   *res_out = rn;
   if (its_out) *its_out = it;

@@ -92,11 +92,38 @@ __global__ void __launch_bounds__(kRedThreads) k_cg_update(T *__restrict__ x, T 
   block_reduce_store<1>(v, part);
 }
 
+// Element-wise vector kernels: thread k takes elements [k W, k W + W), W =
+// 16 / sizeof(T) when every array is 16-byte aligned (one 16-byte access
+// per array and thread), else W = 1; the arithmetic per element is the same
+// either way.
+template <typename T> constexpr int kVecW = 16 / (int)sizeof(T);
+template <typename T, int W> struct alignas(W * sizeof(T)) Lanes {
+  T v[W];
+};
+template <int W, typename T> __device__ __forceinline__ Lanes<T, W> ld(const T *p) {
+  return *reinterpret_cast<const Lanes<T, W> *>(p);
+}
+template <int W, typename T> __device__ __forceinline__ void st(T *p, const Lanes<T, W> &x) {
+  *reinterpret_cast<Lanes<T, W> *>(p) = x;
+}
+inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+template <typename T, int W> inline unsigned vec_blocks(size_t n) {
+  return (unsigned)(((n + W - 1) / W + 255) / 256);
+}
+
 // p = z + beta p
-template <typename T>
+template <typename T, int W>
 __global__ void k_xpby(T *__restrict__ p, const T *__restrict__ z, double beta, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = z[i] + T(beta) * p[i];
+  const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * W;
+  if (i0 + W <= n) {
+    Lanes<T, W> pv = ld<W>(p + i0);
+    const Lanes<T, W> zv = ld<W>(z + i0);
+#pragma unroll
+    for (int k = 0; k < W; ++k) pv.v[k] = zv.v[k] + T(beta) * pv.v[k];
+    st<W>(p + i0, pv);
+  } else {
+    for (size_t i = i0; i < n; ++i) p[i] = z[i] + T(beta) * p[i];
+  }
 }
 
 // y = a x + b y
@@ -107,42 +134,127 @@ __global__ void k_axpby(T *__restrict__ y, const T *__restrict__ x, double a, do
 }
 
 // z = invD .* r
-template <typename T>
+template <typename T, int W>
 __global__ void k_jacobi(T *__restrict__ z, const T *__restrict__ r, const T *__restrict__ invd, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) z[i] = invd[i] * r[i];
+  const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * W;
+  if (i0 + W <= n) {
+    const Lanes<T, W> rv = ld<W>(r + i0), iv = ld<W>(invd + i0);
+    Lanes<T, W> zv;
+#pragma unroll
+    for (int k = 0; k < W; ++k) zv.v[k] = iv.v[k] * rv.v[k];
+    st<W>(z + i0, zv);
+  } else {
+    for (size_t i = i0; i < n; ++i) z[i] = invd[i] * r[i];
+  }
 }
 
 // t = b - t
-template <typename T> __global__ void k_residual(T *__restrict__ t, const T *__restrict__ b, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) t[i] = b[i] - t[i];
+template <typename T, int W> __global__ void k_residual(T *__restrict__ t, const T *__restrict__ b, size_t n) {
+  const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * W;
+  if (i0 + W <= n) {
+    Lanes<T, W> tv = ld<W>(t + i0);
+    const Lanes<T, W> bv = ld<W>(b + i0);
+#pragma unroll
+    for (int k = 0; k < W; ++k) tv.v[k] = bv.v[k] - tv.v[k];
+    st<W>(t + i0, tv);
+  } else {
+    for (size_t i = i0; i < n; ++i) t[i] = b[i] - t[i];
+  }
 }
 
 // Chebyshev first step: d = f2 invD (b - t) (t == nullptr: b), x = x + d or x = d
-template <typename T>
+template <typename T, int W>
 __global__ void k_cheb_first(T *__restrict__ x, T *__restrict__ d, const T *__restrict__ b,
                              const T *__restrict__ t, const T *__restrict__ invd, double f2, int x_is_zero,
                              size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    const T res = t ? b[i] - t[i] : b[i];
-    const T di = T(f2) * invd[i] * res;
-    d[i] = di;
-    x[i] = x_is_zero ? di : x[i] + di;
+  const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * W;
+  if (i0 + W <= n) {
+    const Lanes<T, W> bv = ld<W>(b + i0), iv = ld<W>(invd + i0);
+    Lanes<T, W> tv, xv, dv;
+    if (t) tv = ld<W>(t + i0);
+    if (!x_is_zero) xv = ld<W>(x + i0);
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const T res = t ? bv.v[k] - tv.v[k] : bv.v[k];
+      const T di = T(f2) * iv.v[k] * res;
+      dv.v[k] = di;
+      xv.v[k] = x_is_zero ? di : xv.v[k] + di;
+    }
+    st<W>(d + i0, dv);
+    st<W>(x + i0, xv);
+  } else {
+    for (size_t i = i0; i < n; ++i) {
+      const T res = t ? b[i] - t[i] : b[i];
+      const T di = T(f2) * invd[i] * res;
+      d[i] = di;
+      x[i] = x_is_zero ? di : x[i] + di;
+    }
   }
 }
 
 // Chebyshev recurrence (deal.II's vector_updates): d = f1 d + f2 invD (b - t); x += d
-template <typename T>
+template <typename T, int W>
 __global__ void k_cheb_update(T *__restrict__ x, T *__restrict__ d, const T *__restrict__ b,
                               const T *__restrict__ t, const T *__restrict__ invd, double f1, double f2, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    const T di = T(f1) * d[i] + T(f2) * invd[i] * (b[i] - t[i]);
-    d[i] = di;
-    x[i] += di;
+  const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * W;
+  if (i0 + W <= n) {
+    Lanes<T, W> xv = ld<W>(x + i0), dv = ld<W>(d + i0);
+    const Lanes<T, W> bv = ld<W>(b + i0), tv = ld<W>(t + i0), iv = ld<W>(invd + i0);
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const T di = T(f1) * dv.v[k] + T(f2) * iv.v[k] * (bv.v[k] - tv.v[k]);
+      dv.v[k] = di;
+      xv.v[k] += di;
+    }
+    st<W>(d + i0, dv);
+    st<W>(x + i0, xv);
+  } else {
+    for (size_t i = i0; i < n; ++i) {
+      const T di = T(f1) * d[i] + T(f2) * invd[i] * (b[i] - t[i]);
+      d[i] = di;
+      x[i] += di;
+    }
   }
+}
+
+// host launchers: the 16-byte form when every array allows it
+template <typename T> void launch_xpby(hipStream_t s, T *p, const T *z, double beta, size_t n) {
+  if (aligned16(p) && aligned16(z))
+    hipLaunchKernelGGL((k_xpby<T, kVecW<T>>), dim3(vec_blocks<T, kVecW<T>>(n)), dim3(256), 0, s, p, z, beta, n);
+  else
+    hipLaunchKernelGGL((k_xpby<T, 1>), dim3(vec_blocks<T, 1>(n)), dim3(256), 0, s, p, z, beta, n);
+}
+template <typename T> void launch_jacobi(hipStream_t s, T *z, const T *r, const T *invd, size_t n) {
+  if (aligned16(z) && aligned16(r) && aligned16(invd))
+    hipLaunchKernelGGL((k_jacobi<T, kVecW<T>>), dim3(vec_blocks<T, kVecW<T>>(n)), dim3(256), 0, s, z, r, invd, n);
+  else
+    hipLaunchKernelGGL((k_jacobi<T, 1>), dim3(vec_blocks<T, 1>(n)), dim3(256), 0, s, z, r, invd, n);
+}
+template <typename T> void launch_residual(hipStream_t s, T *t, const T *b, size_t n) {
+  if (aligned16(t) && aligned16(b))
+    hipLaunchKernelGGL((k_residual<T, kVecW<T>>), dim3(vec_blocks<T, kVecW<T>>(n)), dim3(256), 0, s, t, b, n);
+  else
+    hipLaunchKernelGGL((k_residual<T, 1>), dim3(vec_blocks<T, 1>(n)), dim3(256), 0, s, t, b, n);
+}
+template <typename T>
+void launch_cheb_first(hipStream_t s, T *x, T *d, const T *b, const T *t, const T *invd, double f2, int x_is_zero,
+                       size_t n) {
+  if (aligned16(x) && aligned16(d) && aligned16(b) && aligned16(t) && aligned16(invd))
+    hipLaunchKernelGGL((k_cheb_first<T, kVecW<T>>), dim3(vec_blocks<T, kVecW<T>>(n)), dim3(256), 0, s, x, d, b, t,
+                       invd, f2, x_is_zero, n);
+  else
+    hipLaunchKernelGGL((k_cheb_first<T, 1>), dim3(vec_blocks<T, 1>(n)), dim3(256), 0, s, x, d, b, t, invd, f2,
+                       x_is_zero, n);
+}
+template <typename T>
+void launch_cheb_update(hipStream_t s, T *x, T *d, const T *b, const T *t, const T *invd, double f1, double f2,
+                        size_t n) {
+  if (aligned16(x) && aligned16(d) && aligned16(b) && aligned16(t) && aligned16(invd))
+    hipLaunchKernelGGL((k_cheb_update<T, kVecW<T>>), dim3(vec_blocks<T, kVecW<T>>(n)), dim3(256), 0, s, x, d, b, t,
+                       invd, f1, f2, n);
+  else
+    hipLaunchKernelGGL((k_cheb_update<T, 1>), dim3(vec_blocks<T, 1>(n)), dim3(256), 0, s, x, d, b, t, invd, f1, f2,
+                       n);
 }
 
 template <typename TI, typename TO>
