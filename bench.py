@@ -322,63 +322,15 @@ def hanging_record(caching, precision, n=64):
     return out
 
 
-def renumbered_mesh(mesh, perm, order):
-    """the same mesh with node k renamed perm[k] and its cells in the order
-    `order`"""
-    import numpy as np
-    from types import SimpleNamespace
-    coords = np.empty_like(mesh.coords)
-    coords[perm] = mesh.coords
-    return SimpleNamespace(dim=mesh.dim, degree=mesh.degree, n_cells=mesh.n_cells, n_nodes=mesh.n_nodes,
-                           n_dofs=mesh.n_dofs, cell_nodes=np.ascontiguousarray(perm[mesh.cell_nodes[order]]),
-                           coords=coords, cell_colour=None)
-
-
 def permuted_mesh(mesh, seed):
     """the same mesh with its nodes and its cells in a seeded random order
     (no lexicographic numbering left)"""
     import numpy as np
+    import lsmf
     rng = np.random.default_rng(seed)
     perm = rng.permutation(mesh.n_nodes).astype(np.int32)
     order = rng.permutation(mesh.n_cells)
-    return renumbered_mesh(mesh, perm, order), perm, order
-
-
-def cuthill_mckee_mesh(mesh):
-    """the same 3D mesh numbered as the reference numbers its unknowns:
-    nodes in Cuthill-McKee order (DoFRenumbering::Cuthill_McKee,
-    include/mf_elasticity.h:1620) -- here scipy's reverse Cuthill-McKee of
-    the graph joining nodes adjacent along the grid lines, read backwards; the
-    reference orders the full DoF coupling graph, whose wavefronts have
-    another shape -- and cells in Morton order, the order in which a refined
-    hypercube's active cells are visited"""
-    import numpy as np
-    import scipy.sparse as sp
-    from scipy.sparse.csgraph import reverse_cuthill_mckee
-    nn = [c * mesh.degree + 1 for c in mesh.n_cells_dir]
-    idx = np.arange(mesh.n_nodes, dtype=np.int64).reshape(nn[2], nn[1], nn[0])
-    a, b = [], []
-    for ax in range(3):
-        v = np.moveaxis(idx, ax, 0)
-        a.append(v[:-1].ravel())
-        b.append(v[1:].ravel())
-    r, c = np.concatenate(a + b), np.concatenate(b + a)
-    g = sp.csr_matrix((np.ones(len(r), np.int8), (r.astype(np.int32), c.astype(np.int32))),
-                      shape=(mesh.n_nodes, mesh.n_nodes))
-    cm = reverse_cuthill_mckee(g, symmetric_mode=True)[::-1]  # position -> old node
-    perm = np.empty(mesh.n_nodes, np.int32)
-    perm[cm] = np.arange(mesh.n_nodes, dtype=np.int32)
-    del g, r, c, a, b, idx
-    # Morton order of the lexicographic cells
-    nc = mesh.n_cells_dir
-    k = np.arange(mesh.n_cells, dtype=np.int64)
-    xyz = (k % nc[0], (k // nc[0]) % nc[1], k // (nc[0] * nc[1]))
-    code = np.zeros(mesh.n_cells, np.int64)
-    for bit in range(21):
-        for d in range(3):
-            code |= ((xyz[d] >> bit) & 1) << (3 * bit + d)
-    order = np.argsort(code, kind="stable")
-    return renumbered_mesh(mesh, perm, order), perm, order
+    return lsmf.renumber_mesh(mesh, perm, order), perm, order
 
 
 def variant_sweep(cfg_id, reps):
@@ -615,7 +567,7 @@ def main():
             cells["records"].append(rec)
         del pm, perm, pcons, upd
         if mesh.dim == 3:
-            pm, perm, order = cuthill_mckee_mesh(mesh)
+            pm, perm, order = lsmf.cuthill_mckee_numbering(mesh)
             pcons = np.empty_like(cons)
             pcons.reshape(-1, mesh.dim)[perm] = cons.reshape(-1, mesh.dim)
             upd = torch.empty_like(ud)

@@ -15,7 +15,8 @@ import math
 import numpy as np
 
 __all__ = ["SplitMix64", "lame_from_E_nu", "inclusion_material", "lognormal_material",
-           "smooth_displacement", "CONFIGS", "build_config_mesh", "half_refined_box"]
+           "smooth_displacement", "CONFIGS", "build_config_mesh", "half_refined_box", "renumber_mesh",
+           "cuthill_mckee_numbering"]
 
 _M64 = (1 << 64) - 1
 
@@ -233,3 +234,51 @@ def _gll01(p):
     c[-1] = 1
     r = np.sort(L.legroots(L.legder(c)).real)
     return np.concatenate([[0.0], 0.5 * (r + 1), [1.0]])
+
+
+def renumber_mesh(mesh, perm, order):
+    """the same mesh with node k renamed perm[k] and its cells in the order
+    `order` (cell i of the result is cell order[i])"""
+    from types import SimpleNamespace
+    coords = np.empty_like(mesh.coords)
+    coords[perm] = mesh.coords
+    return SimpleNamespace(dim=mesh.dim, degree=mesh.degree, n_cells=mesh.n_cells, n_nodes=mesh.n_nodes,
+                           n_dofs=mesh.n_dofs, cell_nodes=np.ascontiguousarray(perm[mesh.cell_nodes[order]]),
+                           coords=coords, cell_colour=None)
+
+
+def cuthill_mckee_numbering(mesh):
+    """a lexicographic 3D box mesh (lsmf.Mesh) numbered as the reference
+    numbers its unknowns: nodes in Cuthill-McKee order
+    (DoFRenumbering::Cuthill_McKee, include/mf_elasticity.h:1620) -- here
+    scipy's reverse Cuthill-McKee of the graph joining nodes adjacent along
+    the grid lines, read backwards; the reference orders the full DoF
+    coupling graph, whose wavefronts have another shape -- and cells in
+    Morton order, the order in which a refined hypercube's active cells are
+    visited.  Returns (mesh, perm, order) as renumber_mesh takes them."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    assert mesh.dim == 3
+    nn = [c * mesh.degree + 1 for c in mesh.n_cells_dir]
+    idx = np.arange(mesh.n_nodes, dtype=np.int64).reshape(nn[2], nn[1], nn[0])
+    a, b = [], []
+    for ax in range(3):
+        v = np.moveaxis(idx, ax, 0)
+        a.append(v[:-1].ravel())
+        b.append(v[1:].ravel())
+    r, c = np.concatenate(a + b), np.concatenate(b + a)
+    g = sp.csr_matrix((np.ones(len(r), np.int8), (r.astype(np.int32), c.astype(np.int32))),
+                      shape=(mesh.n_nodes, mesh.n_nodes))
+    cm = reverse_cuthill_mckee(g, symmetric_mode=True)[::-1]  # position -> old node
+    perm = np.empty(mesh.n_nodes, np.int32)
+    perm[cm] = np.arange(mesh.n_nodes, dtype=np.int32)
+    del g, r, c, a, b, idx
+    nc = mesh.n_cells_dir
+    k = np.arange(mesh.n_cells, dtype=np.int64)
+    xyz = (k % nc[0], (k // nc[0]) % nc[1], k // (nc[0] * nc[1]))
+    code = np.zeros(mesh.n_cells, np.int64)
+    for bit in range(21):
+        for d in range(3):
+            code |= ((xyz[d] >> bit) & 1) << (3 * bit + d)
+    order = np.argsort(code, kind="stable")
+    return renumber_mesh(mesh, perm, order), perm, order
