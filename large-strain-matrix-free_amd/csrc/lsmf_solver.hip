@@ -213,19 +213,32 @@ template <typename T, int C>
 static void band_launch(const DevBand &b, int d, const int *dims, const T *in, T *out, bool add, hipStream_t s) {
   int od[3] = {dims[0], dims[1], dims[2]};
   od[d] = b.n_out;
-  if (od[1] > 65535 || od[2] > 65535) throw HipError(LSMF_ERR_UNSUPPORTED, "transfer: more than 65535 nodes per direction");
-  const dim3 grid((unsigned)((od[0] + 255) / 256), (unsigned)od[1], (unsigned)od[2]);
   const T *wt = b.weights<T>();
   const int a = add ? 1 : 0;
-  if (d == 0)
+  constexpr int R = 4;  // outputs per lane along y / z
+  const size_t line_bytes = (size_t)dims[0] * C * sizeof(T);
+  if (d == 0 && line_bytes <= 65536 && (size_t)dims[1] * dims[2] <= (size_t)INT32_MAX) {
+    hipLaunchKernelGGL((k_band_line<T, C>), dim3((unsigned)((size_t)dims[1] * dims[2])), dim3(256), line_bytes, s, in,
+                       out, b.start.p, wt, b.W, dims[0], b.n_out, a);
+  } else if (d == 0) {
+    if (od[2] > 65535 || (size_t)od[0] * od[1] > (size_t)INT32_MAX)
+      throw HipError(LSMF_ERR_UNSUPPORTED, "transfer: level grid too large");
+    const dim3 grid((unsigned)(((size_t)od[0] * od[1] + 255) / 256), (unsigned)od[2]);
     hipLaunchKernelGGL((k_band<T, 0, C>), grid, dim3(256), 0, s, in, out, b.start.p, b.len.p, wt, b.W, dims[0], dims[1],
                        b.n_out, a);
-  else if (d == 1)
-    hipLaunchKernelGGL((k_band<T, 1, C>), grid, dim3(256), 0, s, in, out, b.start.p, b.len.p, wt, b.W, dims[0], dims[1],
-                       b.n_out, a);
-  else
-    hipLaunchKernelGGL((k_band<T, 2, C>), grid, dim3(256), 0, s, in, out, b.start.p, b.len.p, wt, b.W, dims[0], dims[1],
-                       b.n_out, a);
+  } else {
+    const size_t other = (size_t)dims[0] * (d == 1 ? dims[2] : dims[1]);
+    const int oblocks = (b.n_out + R - 1) / R;
+    if (oblocks > 65535 || other > (size_t)INT32_MAX)
+      throw HipError(LSMF_ERR_UNSUPPORTED, "transfer: level grid too large");
+    const dim3 grid((unsigned)((other + 255) / 256), (unsigned)oblocks);
+    if (d == 1)
+      hipLaunchKernelGGL((k_band_rows<T, 1, C, R>), grid, dim3(256), 0, s, in, out, b.start.p, b.len.p, wt, b.W,
+                         dims[0], dims[1], dims[2], b.n_out, a);
+    else
+      hipLaunchKernelGGL((k_band_rows<T, 2, C, R>), grid, dim3(256), 0, s, in, out, b.start.p, b.len.p, wt, b.W,
+                         dims[0], dims[1], dims[2], b.n_out, a);
+  }
 }
 
 // apply one band per direction (tensor product), in has node dims nin[0..dim)

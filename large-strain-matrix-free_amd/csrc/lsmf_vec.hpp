@@ -461,12 +461,15 @@ template <typename T, int DIR, int C>
 __global__ void __launch_bounds__(256) k_band(const T *__restrict__ in, T *__restrict__ out,
                                               const int32_t *__restrict__ start, const int32_t *__restrict__ len,
                                               const T *__restrict__ w, int W, int n0, int n1, int n_out, int add) {
+  // output node (i0, i1, i2): (i0, i1) flattened over grid x, so that short
+  // lines (257 nodes) fill whole workgroups; i2 = grid y
   const int m0 = DIR == 0 ? n_out : n0, m1 = DIR == 1 ? n_out : n1;
-  const int i0 = blockIdx.x * 256 + threadIdx.x;
-  if (i0 >= m0) return;
-  const int o = DIR == 0 ? i0 : (DIR == 1 ? (int)blockIdx.y : (int)blockIdx.z);
+  const unsigned idx = blockIdx.x * 256u + threadIdx.x;
+  if (idx >= (unsigned)m0 * (unsigned)m1) return;
+  const int i1 = (int)(idx / (unsigned)m0), i0 = (int)idx - i1 * m0, i2 = (int)blockIdx.y;
+  const int o = DIR == 0 ? i0 : (DIR == 1 ? i1 : i2);
   const int s = start[o];
-  const int j0 = DIR == 0 ? s : i0, j1 = DIR == 1 ? s : (int)blockIdx.y, j2 = DIR == 2 ? s : (int)blockIdx.z;
+  const int j0 = DIR == 0 ? s : i0, j1 = DIR == 1 ? s : i1, j2 = DIR == 2 ? s : i2;
   const size_t line = (size_t)n0 * C;
   const T *src = in + ((size_t)j2 * n1 + j1) * line + (size_t)j0 * C;
   const size_t stride = DIR == 0 ? (size_t)C : (DIR == 1 ? line : line * n1);
@@ -482,7 +485,7 @@ __global__ void __launch_bounds__(256) k_band(const T *__restrict__ in, T *__res
 #pragma unroll
     for (int c = 0; c < C; ++c) acc[c] = fma(wk, x.v[c], acc[c]);
   }
-  T *dst = out + (((size_t)blockIdx.z * m1 + blockIdx.y) * m0 + i0) * C;
+  T *dst = out + (((size_t)i2 * m1 + i1) * m0 + i0) * C;
   BandNode<C, T> r;
 #pragma unroll
   for (int c = 0; c < C; ++c) r.v[c] = acc[c];
@@ -493,6 +496,117 @@ __global__ void __launch_bounds__(256) k_band(const T *__restrict__ in, T *__res
     for (int c = 0; c < C; ++c) r.v[c] += d.v[c];
   }
   __builtin_memcpy(dst, &r, sizeof(r));
+}
+
+
+// Banded 1D operator along x (DIR 0) with the input line staged in LDS: one
+// workgroup per line (i1, i2) reads its n0 nodes once, coalesced, and each
+// lane then computes output nodes o = t, t + 256, ... from LDS (the W taps
+// from start[o]; rows are padded to W taps inside the line).  Same
+// arithmetic as k_band<T, 0, C>.  Dynamic LDS: n0 * C values.
+template <typename T, int C>
+__global__ void __launch_bounds__(256) k_band_line(const T *__restrict__ in, T *__restrict__ out,
+                                                   const int32_t *__restrict__ start, const T *__restrict__ w, int W,
+                                                   int n0, int n_out, int add) {
+  extern __shared__ unsigned char band_smem[];
+  T *buf = reinterpret_cast<T *>(band_smem);
+  const size_t line = blockIdx.x;
+  const T *src = in + line * (size_t)n0 * C;
+  for (int e = threadIdx.x; e < n0 * C; e += 256) buf[e] = src[e];
+  __syncthreads();
+  T *dst = out + line * (size_t)n_out * C;
+  for (int o = threadIdx.x; o < n_out; o += 256) {
+    const T *b = buf + (size_t)start[o] * C;
+    const T *wr = w + (size_t)o * W;
+    T acc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) acc[c] = T(0);
+    for (int k = 0; k < W; ++k) {
+      const T wk = wr[k];
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc[c] = fma(wk, b[k * C + c], acc[c]);
+    }
+    BandNode<C, T> r;
+#pragma unroll
+    for (int c = 0; c < C; ++c) r.v[c] = acc[c];
+    if (add) {
+      BandNode<C, T> d;
+      __builtin_memcpy(&d, dst + (size_t)o * C, sizeof(d));
+#pragma unroll
+      for (int c = 0; c < C; ++c) r.v[c] += d.v[c];
+    }
+    __builtin_memcpy(dst + (size_t)o * C, &r, sizeof(r));
+  }
+}
+
+// Banded 1D operator along y (DIR 1) or z (DIR 2), R consecutive outputs
+// per lane: the lanes of a workgroup cover nodes of the two other
+// directions (grid x) and share the outputs o0 = R blockIdx.y, ...,
+// o0 + R - 1, so every input row of their bands' union is loaded once
+// (coalesced along x) instead of once per output; the band bookkeeping is
+// uniform over the workgroup.  Each output sums its taps in the order of
+// k_band<T, DIR, C>.
+template <typename T, int DIR, int C, int R>
+__global__ void __launch_bounds__(256) k_band_rows(const T *__restrict__ in, T *__restrict__ out,
+                                                   const int32_t *__restrict__ start,
+                                                   const int32_t *__restrict__ len, const T *__restrict__ w, int W,
+                                                   int n0, int n1, int n2, int n_out, int add) {
+  static_assert(DIR == 1 || DIR == 2, "rows of y or z");
+  const unsigned idx = blockIdx.x * 256u + threadIdx.x;
+  const int nb = DIR == 1 ? n2 : n1;  // the other direction flattened with x
+  if (idx >= (unsigned)n0 * (unsigned)nb) return;
+  const int ib = (int)(idx / (unsigned)n0), i0 = (int)idx - ib * n0;
+  const int o0 = (int)blockIdx.y * R, nr = min(R, n_out - o0);
+  int st[R], ln[R];
+  int jlo = start[o0], jhi = jlo;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    st[r] = r < nr ? start[o0 + r] : 0;
+    ln[r] = r < nr ? len[o0 + r] : 0;
+    if (r < nr) {
+      jlo = min(jlo, st[r]);
+      jhi = max(jhi, st[r] + ln[r]);
+    }
+  }
+  // input node (i0, j, ib) or (i0, ib, j); element stride between rows
+  const size_t row = DIR == 1 ? (size_t)n0 * C : (size_t)n0 * n1 * C;
+  const T *src = in + (DIR == 1 ? (size_t)ib * n1 * n0 * C : (size_t)ib * n0 * C) + (size_t)i0 * C;
+  T acc[R][C];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int c = 0; c < C; ++c) acc[r][c] = T(0);
+  for (int j = jlo; j < jhi; ++j) {
+    BandNode<C, T> x;
+    __builtin_memcpy(&x, src + (size_t)j * row, sizeof(x));
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = j - st[r];
+      if (k >= 0 && k < ln[r]) {
+        const T wk = w[(size_t)(o0 + r) * W + k];
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[r][c] = fma(wk, x.v[c], acc[r][c]);
+      }
+    }
+  }
+  // output node (i0, o, ib) or (i0, ib, o)
+  const size_t orow = DIR == 1 ? (size_t)n0 * C : (size_t)n0 * n1 * C;
+  T *dst = out + (DIR == 1 ? (size_t)ib * n_out * n0 * C : (size_t)ib * n0 * C) + (size_t)i0 * C;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (r >= nr) break;
+    T *d = dst + (size_t)(o0 + r) * orow;
+    BandNode<C, T> v;
+#pragma unroll
+    for (int c = 0; c < C; ++c) v.v[c] = acc[r][c];
+    if (add) {
+      BandNode<C, T> old;
+      __builtin_memcpy(&old, d, sizeof(old));
+#pragma unroll
+      for (int c = 0; c < C; ++c) v.v[c] += old.v[c];
+    }
+    __builtin_memcpy(d, &v, sizeof(v));
+  }
 }
 
 }  // namespace lsmf
