@@ -617,6 +617,11 @@ def main():
         progress(f"  gradient {gradient}")
         progress("caching-variant sweep on config 1 (32^3 Q2)")
         variants = variant_sweep(1, 20)
+        if not args.no_solve:
+            progress("config 1: 5-load-step Newton solve (32^3 Q2, tensor4)")
+            m1, mu1, lam1 = lsmf.build_config_mesh(1)
+            variants["newton_solve"] = solve_pair(1, None, mu1, lam1, args.solve_tol, args.coarse)
+            del m1
 
     cpu = None
     if not args.no_cpu:

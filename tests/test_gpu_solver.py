@@ -383,6 +383,36 @@ def test_config2_problem_fp32_levels_against_fp64_oracle(device):
     assert err < 1e-7
 
 
+def test_config1_problem_against_fp64_oracle(device):
+    """the 32^3 configuration's problem (nu = 0.3, 20 inclusions of radius
+    0.08 and 10x stiffness, tensor4, x=0 clamped, 5 load steps of u_x =
+    0.01, GMG-CG to 1e-8, FP64 levels) on 8^3 cells against the oracle's
+    Newton (Jacobi CG): Newton counts within 1 and the displacement to 1e-7
+    of its maximum"""
+    log("[config1 8^3] FP64-GMG Newton vs FP64 oracle")
+    cfg = lsmf.CONFIGS[1]
+    mesh, mu, lam = lsmf.build_config_mesh(1, n=8)
+    o = lsmf.solver_options(dim=3, degree=2, n_q_1d=3, n_cells_coarse=[2, 2, 2], n_levels=3, caching="tensor4",
+                            mg_caching="tensor4", mg_precision="fp64", tol_lin=1e-8, tol_f=1e-8, dt=1.0 / 5,
+                            t_end=1.0 + 1e-9, coarse_solver="direct",
+                            dirichlet={0: ([0, 1, 2], [0.0] * 3), 1: ([0], [cfg["increment"]])})
+    S = lsmf.Solver(o, mu, lam)
+    stats = S.run()
+    u = S.solution()
+    assert len(stats) == 5
+    cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
+    inc = np.zeros(mesh.n_dofs)
+    inc[mesh.boundary_nodes(1) * 3] = cfg["increment"]
+    ref = oracle.OracleOperator(3, 2, mesh.cell_nodes, mesh.coords, cons, caching="tensor4")
+    ref.set_material(mu, lam)
+    its, _, u_ref = ref.newton(1.0 / 5, 1.0 + 1e-9, dirichlet_inc=inc, tol_f=1e-8, tol_lin=1e-8)
+    err = float(np.abs(u - u_ref).max() / np.abs(u_ref).max())
+    log(f"  GPU Newton {[s['newton_iterations'] for s in stats]} oracle {list(its)}, |u - u_oracle| / |u| = {err:.2e}")
+    assert len(its) == 5
+    assert all(abs(a["newton_iterations"] - b) <= 1 for a, b in zip(stats, its))
+    assert err < 1e-7
+
+
 def test_config4_problem_fp32_levels_against_fp64_oracle(device):
     """the 64^3 Q4 configuration's problem (nu = 0.45, log-normal mu/lambda
     with sigma = 0.3, tensor2, x=0 clamped, 3 load steps of u_x = 0.03, CG
