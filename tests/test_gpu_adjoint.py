@@ -210,35 +210,41 @@ def test_misfit_gradient_through_newton_solve(degree, caching, mg_precision, sig
         assert abs(fd - an) < 1e-5 * abs(an)
 
 
-def test_config3_misfit_gradient_at_full_size(device):
-    """the differentiation workload at its size: 48^3 Q2 (2,738,019 DoF),
-    log-normal mu, lambda (sigma = 0.5), x=0 clamped, u_x = 0.01 at x=1 in
-    one load step, FP64 levels; J = misfit of the x=1 face displacement
-    against the solution for a second seeded field.  dJ/dmu, dJ/dlambda
-    from one adjoint solve + one VJP against central differences of two
-    full Newton solves (h = 1e-5) on 2 seeded directions, to 1e-5
-    relative"""
-    cfg = lsmf.CONFIGS[3]
-    n = cfg["n"]
-    log(f"[adjoint solve] {n}^3 Q2 (config 3 at full size)")
-    mesh = lsmf.Mesh(3, 2, [n] * 3)
-    assert mesh.n_dofs == 2738019
+@pytest.mark.parametrize("cfg_id", [3, 4])
+def test_misfit_gradient_at_full_size(cfg_id, device):
+    """the differentiation workload at full size -- 48^3 Q2 tensor4
+    (2,738,019 DoF, sigma = 0.5) and the 64^3 Q4 tensor2 configuration
+    (50,923,779 DoF, sigma = 0.3) -- with log-normal mu, lambda, x=0
+    clamped, u_x = 0.01 at x=1 in one load step, FP64 levels; J = misfit of
+    the x=1 face displacement against the solution for a second seeded
+    field.  dJ/dmu, dJ/dlambda from one adjoint solve + one VJP against
+    central differences of two full Newton solves (h = 1e-5) on 2 seeded
+    directions, to 1e-5 relative"""
+    cfg = lsmf.CONFIGS[cfg_id]
+    n, p = cfg["n"], cfg["degree"]
+    log(f"[adjoint solve] {n}^3 Q{p} (config {cfg_id} at full size)")
+    mesh = lsmf.Mesh(3, p, [n] * 3)
+    assert mesh.n_dofs == {3: 2738019, 4: 50923779}[cfg_id]
     cons = mesh.constrained_mask({0: [0, 1, 2], 1: [0]})
-    o = lsmf.solver_options(dim=3, degree=2, n_q_1d=3, n_cells_coarse=[3] * 3, n_levels=5, caching="tensor4",
-                            mg_caching="tensor4", mg_precision="fp64", coarse_solver="direct", tol_lin=1e-12,
+    nc = 3 if n % 3 == 0 else 2
+    o = lsmf.solver_options(dim=3, degree=p, n_q_1d=p + 1, n_cells_coarse=[nc] * 3,
+                            n_levels=int(round(np.log2(n // nc))) + 1, caching=cfg["caching"],
+                            mg_caching=cfg["caching"], mg_precision="fp64", coarse_solver="direct", tol_lin=1e-12,
                             tol_f=1e-13, tol_u=1e-12, dt=1.0, t_end=1.0,
                             dirichlet={0: ([0, 1, 2], [0.0] * 3), 1: ([0], [0.01])})
 
-    def solve(mu, lam):
+    def solve(mu, lam, what):
         S = lsmf.Solver(o, mu, lam)
-        S.run()
+        stats = S.run()
+        log(f"  solve {what}: Newton {[s['newton_iterations'] for s in stats]}")
         return S, S.solution()
 
     seed = lsmf.workload.SEED
     mu_t, lam_t = lsmf.lognormal_material(mesh.n_cells, 1.0, cfg["nu"], cfg["sigma"], seed + 31)
-    _, target = solve(mu_t, lam_t)
-    mu, lam = lsmf.lognormal_material(mesh.n_cells, 1.0, cfg["nu"], cfg["sigma"], seed + 3)
-    S, u_star = solve(mu, lam)
+    S, target = solve(mu_t, lam_t, "target")
+    del S
+    mu, lam = lsmf.lognormal_material(mesh.n_cells, 1.0, cfg["nu"], cfg["sigma"], seed + cfg_id)
+    S, u_star = solve(mu, lam, "forward")
     face = mesh.boundary_nodes(1)
     J, gmu, gla = misfit_gradient(S, mesh, u_star, mu, lam, target, face, cons)
     del S
@@ -252,7 +258,7 @@ def test_config3_misfit_gradient_at_full_size(device):
         h = 1e-5
         Js = []
         for sgn in (1.0, -1.0):
-            S2, us = solve(mu + sgn * h * b, lam + sgn * h * c)
+            S2, us = solve(mu + sgn * h * b, lam + sgn * h * c, f"direction {k} {'+' if sgn > 0 else '-'}")
             del S2
             d = us[dofs] - target[dofs]
             Js.append(0.5 * float(d @ d))
