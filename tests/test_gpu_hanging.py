@@ -138,3 +138,33 @@ def test_half_refined_box_matches_oracle_and_is_deterministic(p, caching, device
     pl = torch.full_like(src, np.nan)
     op.vmult(pl, src, mask="plain")
     assert rel(pl.cpu().numpy(), ref.vmult(v, plain=True)) < 1e-12
+
+
+def test_half_refined_box_batched_beyond_one_result_buffer(device):
+    """batched products with constraints run in groups of the 8 vectors the
+    constrained-node result buffer holds: 11 vectors give the 11 single
+    products bit for bit"""
+    import torch
+    log("[hanging] half_refined_box batched, 11 vectors")
+    mesh, nc = lsmf.half_refined_box(2, 2)
+    rng = np.random.default_rng(13)
+    mu, lam = 1.0 + rng.random(mesh.n_cells), 1.5 + rng.random(mesh.n_cells)
+    C = hanging_mesh.constraint_matrix(mesh, nc)
+    x = mesh.coords
+    u = C @ (0.02 * np.sin(np.pi * (x @ np.arange(1, 4)))[:, None] * np.ones(3)).reshape(-1)
+    cons = np.zeros(mesh.n_dofs, np.uint8)
+    left = np.where(np.abs(x[:, 0]) < 1e-12)[0]
+    for c in range(3):
+        cons[left * 3 + c] = 1
+    op = lsmf.NeoHookOperator(mesh, precision="fp64", caching="tensor4", constrained=cons, use_mesh_colouring=False,
+                              node_constraints=nc)
+    op.set_material(mu, lam)
+    op.set_linearization(torch.tensor(u, device=device))
+    op.cache()
+    X = torch.tensor(rng.standard_normal((11, mesh.n_dofs)), device=device)
+    Y = torch.full_like(X, np.nan)
+    op.vmult_batched(Y, X)
+    for b in range(11):
+        y = torch.full_like(X[b], np.nan)
+        op.vmult(y, X[b].contiguous())
+        assert torch.equal(Y[b], y), b

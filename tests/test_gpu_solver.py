@@ -40,11 +40,16 @@ def tensor_poly(coords, p):
     return f.reshape(-1)
 
 
-@pytest.mark.parametrize("dim,p", [(2, 1), (2, 2), (3, 2), (3, 4)])
-def test_transfer_exactness(dim, p, device):
+@pytest.mark.parametrize("dim,p,nc", [(2, 1, (2, 3)), (2, 2, (2, 3)), (3, 2, (2, 3, 2)), (3, 4, (2, 3, 2)),
+                                      (3, 2, (350, 1, 1))])
+def test_transfer_exactness(dim, p, nc, device):
+    """prolongation reproduces Q_p fields, interpolation inverts it and
+    restriction is its transpose, on every level; the 1400x4x4 mesh's
+    x-lines (2,801 nodes, 67 KB in FP64) exceed what the x pass stages in
+    LDS and take its global-memory form"""
     import torch
-    log(f"[transfer] dim={dim} p={p}")
-    nc = [2, 3, 2][:dim]
+    log(f"[transfer] dim={dim} p={p} nc={nc}")
+    nc = list(nc)
     o = lsmf.solver_options(dim=dim, degree=p, n_q_1d=p + 1, n_cells_coarse=nc, n_levels=3, caching="tensor4",
                             mg_caching="tensor4", mg_precision="fp64")
     mesh = lsmf.Mesh(dim, p, [k * 4 for k in nc])
