@@ -72,6 +72,17 @@ void Reducer::dot2(const T *a, const T *b, const T *c, const T *d, size_t n, dou
   r1 = r[1];
 }
 
+template <typename TA, typename TB> void Reducer::dot_to(const TA *a, const TB *b, size_t n, double *dst) {
+  hipLaunchKernelGGL((k_dot<TA, TB>), dim3(kRedBlocks), dim3(kRedThreads), 0, s_, a, b, n, part_.p);
+  hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kRedThreads), 0, s_, part_.p, dst);
+}
+
+template <typename T>
+void Reducer::cg_update_dev(T *x, T *r, const T *p, const T *Ap, const double *st, size_t n, double *dst) {
+  hipLaunchKernelGGL(k_cg_update_dev<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, s_, x, r, p, Ap, st, n, part_.p);
+  hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kRedThreads), 0, s_, part_.p, dst);
+}
+
 template <typename T> double Reducer::cg_update(T *x, T *r, const T *p, const T *Ap, double alpha, size_t n) {
   hipLaunchKernelGGL(k_cg_update<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, s_, x, r, p, Ap, alpha, n, part_.p);
   double rr[2];
@@ -670,29 +681,37 @@ template <typename T> class GMG : public GMGBase {
     const size_t n = v.g->n_dofs;
     T *x = v.sol.p, *r = v.rhs.p, *p = v.d.p, *Ap = v.t.p, *z = v.z.p;
     const T *invd = v.op->get_matrix_diagonal_inverse();
+    if (eig_ab_.n < (size_t)2 * its) eig_ab_.resize((size_t)2 * its);
+    double *st = eig_st_.p, *al = eig_ab_.p, *be = eig_ab_.p + its;
     hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, s_, x, T(0), n);
     hipLaunchKernelGGL(k_pseudo_random<T>, dim3(nblk(n)), dim3(256), 0, s_, r, n, (uint64_t)20261202ull);
     zero_constrained((int)(&v - lv_.data()), r);
     launch_jacobi<T>(s_, z, (const T *)r, invd, n);
     hip_check(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, s_), "eig copy");
-    double rz = red_.dot(r, z, n);
-    const double r0 = std::sqrt(red_.dot(r, r, n));
-    std::vector<double> alpha, beta;
+    red_.dot_to(r, z, n, st + kEigRz);
+    red_.dot_to(r, r, n, st + kEigR0);
+    hipLaunchKernelGGL(k_eig_init, dim3(1), dim3(1), 0, s_, st);
+    // the CG steps with their scalars on the device; the recording stops at
+    // a non-positive p.Ap or once the residual is below 1e-10 r0, and the
+    // remaining steps' updates are then no-ops
     for (int k = 0; k < its; ++k) {
       v.op->vmult(Ap, p);
-      const double pAp = red_.dot(p, Ap, n);
-      if (!(pAp > 0)) break;
-      const double a = rz / pAp;
-      alpha.push_back(a);
-      const double rr = red_.cg_update(x, r, p, Ap, a, n);
-      if (std::sqrt(rr) <= 1e-10 * r0) break;
+      red_.dot_to(p, Ap, n, st + kEigPAp);
+      hipLaunchKernelGGL(k_eig_alpha, dim3(1), dim3(1), 0, s_, st, al, k);
+      red_.cg_update_dev(x, r, p, Ap, st, n, st + kEigRr);
+      hipLaunchKernelGGL(k_eig_check, dim3(1), dim3(1), 0, s_, st);
       launch_jacobi<T>(s_, z, (const T *)r, invd, n);
-      const double rz_new = red_.dot(r, z, n);
-      const double b = rz_new / rz;
-      beta.push_back(b);
-      rz = rz_new;
-      launch_xpby<T>(s_, p, (const T *)z, b, n);
+      red_.dot_to(r, z, n, st + kEigRzNew);
+      hipLaunchKernelGGL(k_eig_beta, dim3(1), dim3(1), 0, s_, st, be, k);
+      hipLaunchKernelGGL(k_xpby_dev<T>, dim3(nblk(n)), dim3(256), 0, s_, p, (const T *)z, (const double *)st, n);
     }
+    std::vector<double> hst(kEigSlots), hab((size_t)2 * its);
+    hip_check(hipMemcpyAsync(hst.data(), st, kEigSlots * sizeof(double), hipMemcpyDeviceToHost, s_), "eig read");
+    hip_check(hipMemcpyAsync(hab.data(), eig_ab_.p, (size_t)2 * its * sizeof(double), hipMemcpyDeviceToHost, s_),
+              "eig read");
+    hip_check(hipStreamSynchronize(s_), "eig sync");
+    const int mm = (int)hst[kEigM];
+    std::vector<double> alpha(hab.begin(), hab.begin() + mm), beta(hab.begin() + its, hab.begin() + its + mm);
     const size_t m = alpha.size();
     if (m == 0) throw HipError(LSMF_ERR_HIP, "eigenvalue estimate: no CG step");
     std::vector<double> diag(m), off(m > 1 ? m - 1 : 0);
@@ -711,6 +730,7 @@ template <typename T> class GMG : public GMGBase {
   lsmf_solver_options opt_;
   hipStream_t s_;
   Reducer red_;
+  DeviceArray<double> eig_st_{kEigSlots}, eig_ab_;  // device-resident eigenvalue-estimate CG
   std::vector<Level<T>> lv_;
   size_t tmax64_ = 0;
   double cache_s_ = 0.0;

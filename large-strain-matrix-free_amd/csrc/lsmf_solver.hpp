@@ -147,6 +147,10 @@ class Reducer {
   template <typename TA, typename TB> double dot(const TA *a, const TB *b, size_t n);
   template <typename T> void dot2(const T *a, const T *b, const T *c, const T *d, size_t n, double &r0, double &r1);
   template <typename T> double cg_update(T *x, T *r, const T *p, const T *Ap, double alpha, size_t n);
+  // the same reductions with the result left in device memory (no host sync)
+  template <typename TA, typename TB> void dot_to(const TA *a, const TB *b, size_t n, double *dst);
+  template <typename T> void cg_update_dev(T *x, T *r, const T *p, const T *Ap, const double *st, size_t n,
+                                           double *dst);
 
  private:
   double read(int nv, double *out);

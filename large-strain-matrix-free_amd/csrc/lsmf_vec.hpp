@@ -92,6 +92,70 @@ __global__ void __launch_bounds__(kRedThreads) k_cg_update(T *__restrict__ x, T 
   block_reduce_store<1>(v, part);
 }
 
+// Device-resident CG for the Lanczos eigenvalue estimate: its scalars live
+// in a small device state, so a step needs no host round trip.  State
+// slots (doubles): rz, r0, alpha and beta of the current step, the number
+// of recorded steps m and a stop flag (a non-positive p.Ap, or the residual
+// below 1e-10 r0, ends the recording, as the host loop's breaks did); the
+// reductions land in their own slots.  Same arithmetic as the host loop.
+enum EigSlot { kEigRz = 0, kEigR0, kEigAlpha, kEigBeta, kEigM, kEigStop, kEigPAp, kEigRr, kEigRzNew, kEigSlots };
+
+__global__ void k_eig_init(double *__restrict__ st) {
+  st[kEigR0] = sqrt(st[kEigR0]);  // holds r.r on entry
+  st[kEigM] = 0.0;
+  st[kEigStop] = 0.0;
+}
+// alpha_k = rz / p.Ap
+__global__ void k_eig_alpha(double *__restrict__ st, double *__restrict__ alpha, int k) {
+  if (st[kEigStop] != 0.0) return;
+  const double pAp = st[kEigPAp];
+  if (!(pAp > 0)) {
+    st[kEigStop] = 1.0;
+    return;
+  }
+  const double a = st[kEigRz] / pAp;
+  alpha[k] = a;
+  st[kEigAlpha] = a;
+  st[kEigM] = k + 1;
+}
+__global__ void k_eig_check(double *__restrict__ st) {
+  if (st[kEigStop] != 0.0) return;
+  if (sqrt(st[kEigRr]) <= 1e-10 * st[kEigR0]) st[kEigStop] = 1.0;
+}
+// beta_k = rz_new / rz
+__global__ void k_eig_beta(double *__restrict__ st, double *__restrict__ beta, int k) {
+  if (st[kEigStop] != 0.0) return;
+  const double b = st[kEigRzNew] / st[kEigRz];
+  beta[k] = b;
+  st[kEigBeta] = b;
+  st[kEigRz] = st[kEigRzNew];
+}
+// k_cg_update with alpha from the state; no-op once the recording stopped
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_cg_update_dev(T *__restrict__ x, T *__restrict__ r,
+                                                               const T *__restrict__ p, const T *__restrict__ Ap,
+                                                               const double *__restrict__ st, size_t n,
+                                                               double *__restrict__ part) {
+  if (st[kEigStop] != 0.0) return;
+  const double alpha = st[kEigAlpha];
+  double v[1] = {0.0};
+  for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads) {
+    x[i] = x[i] + T(alpha) * p[i];
+    const T ri = r[i] - T(alpha) * Ap[i];
+    r[i] = ri;
+    v[0] += double(ri) * double(ri);
+  }
+  block_reduce_store<1>(v, part);
+}
+// p = z + beta p with beta from the state; no-op once the recording stopped
+template <typename T>
+__global__ void k_xpby_dev(T *__restrict__ p, const T *__restrict__ z, const double *__restrict__ st, size_t n) {
+  if (st[kEigStop] != 0.0) return;
+  const double beta = st[kEigBeta];
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + T(beta) * p[i];
+}
+
 // Element-wise vector kernels: thread k takes elements [k W, k W + W), W =
 // 16 / sizeof(T) when every array is 16-byte aligned (one 16-byte access
 // per array and thread), else W = 1; the arithmetic per element is the same
