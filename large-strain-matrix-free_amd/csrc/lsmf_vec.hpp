@@ -506,18 +506,13 @@ __global__ void __launch_bounds__(256) k_bgj_update(double *__restrict__ A, int 
 // (n2, n1, n0, C) (x fastest, C components interleaved); the output has
 // n_DIR replaced by n_out:
 //   out[.., o, ..] = sum_{k < len[o]} w[o * W + k] in[.., start[o] + k, ..]
-// (add != 0: out +=, the prolongate-and-add of the V-cycle).  One thread
-// per output node: its C components come in and go out as one access per
-// tap (12 bytes in FP32, 24 in FP64), and each tap's weight is loaded once
-// for the C components -- a thread per component loaded every weight C
-// times and every value with its own narrow load, and was bound by the
-// number of load instructions, not by HBM.  Grid: x covers the m0 nodes of
-// one x-line, 256 per block; y and z are the output's i1 and i2 (no index
-// division).  Along y and z the row o is the same for the whole block (its
-// start, length and weights are scalar loads) and the taps loop over its
-// nonzero length; along x the lanes differ in o and loop over the padded
-// width W (zero weights past a row's length).  T is the level's precision:
-// FP32 levels transfer in FP32 with FP32 weights.
+// (add != 0: out +=, the prolongate-and-add of the V-cycle).  T is the
+// level's precision: FP32 levels transfer in FP32 with FP32 weights.  The
+// transfer runs k_band_line (x) and k_band_rows (y, z) below; this form,
+// one thread per output node with its C components loaded as one access
+// per tap, serves x-lines too long to stage in LDS.  Grid: x covers the
+// output nodes (i0, i1) flattened, y is i2.  Along x the lanes differ in o
+// and loop over the padded width W (zero weights past a row's length).
 template <int C, typename T> struct BandNode {
   T v[C];
 };
