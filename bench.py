@@ -17,14 +17,19 @@ doc/large_strain.tex:739-753, divided by the kernel's event-timed duration)
 and a CPU baseline timed on the host.  Beside the headline it reports
   * the other caching variants and precisions on the same 128^3 mesh, each
     against its roofline (scalar also against the FP64 VALU peak),
-  * renumbered meshes (the reference's DoF renumbering): the cells layout
-    on the lexicographic and on a randomly renumbered mesh, and the tiles
-    the operator recovers from the cells' face neighbours on the latter,
+  * renumbered meshes: the cells layout on the lexicographic numbering, and
+    the cells layout and the tiles the operator recovers from the cells'
+    face neighbours on a Cuthill-McKee numbering (the reference's DoF
+    renumbering) and on a random one,
   * the FP32-V-cycle against the all-FP64 Newton solve (3 load steps, CG to
-    1e-8) on the 128^3 Q2 and the 64^3 Q4 meshes,
-  * the 64^3 Q4 tensor2 configuration's vmult and roofline,
+    1e-8) on the 128^3 Q2 and the 64^3 Q4 meshes, and the 32^3 Q2 5-step
+    solve,
+  * the 64^3 Q4 tensor2 configuration's vmult and roofline, its residual VJP
+    and the misfit gradient through its Newton solve,
   * the 48^3 Q2 differentiation workload: residual and its VJP, and the
-    gradient of the x = 1 face misfit through a Newton solve.
+    gradient of the x = 1 face misfit through a Newton solve,
+  * hanging-node constraints on a half-refined cube (vmult and diagonal with
+    and without the constraints).
 """
 import argparse
 import hashlib
